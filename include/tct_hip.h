@@ -1,0 +1,190 @@
+/*
+ * tct_hip.h -- C ABI of libtct_hip.so, the MI355X (gfx950) farfield /
+ * nearfield hot path of tectosaur.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a negative TCT_E* code on failure; the message of
+ *     the last failure on the calling thread is in tct_last_error();
+ *   - every pointer marked "dev" is device memory owned by the caller;
+ *     every output is caller-allocated; nothing here allocates device memory
+ *     except where a comment says so;
+ *   - "stream" is a hipStream_t passed as void*; work is enqueued on it and
+ *     the call returns without synchronising (except where noted);
+ *   - indices are int32 on the device, int64 on the host.
+ *
+ * Kernel ids follow the reference's kernel names (tectosaur/kernels/
+ * __init__.py:20-258).  DOF layout is v[9*tri + 3*basis + component]
+ * (tectosaur/matrix_free.cl:53-56); pair blocks are
+ * [b_obs*27 + d_obs*9 + b_src*3 + d_src] (tectosaur/assemble.cl:94-97).
+ */
+#ifndef TCT_HIP_H
+#define TCT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TCT_OK 0
+#define TCT_EARG (-1)
+#define TCT_EHIP (-2)
+#define TCT_EUNSUPPORTED (-3)
+
+enum tct_kernel {
+    TCT_U = 0,  /* elasticU3  */
+    TCT_T = 1,  /* elasticT3  */
+    TCT_A = 2,  /* elasticA3  */
+    TCT_H = 3,  /* elasticH3  */
+    TCT_RT = 4, /* elasticRT3 */
+    TCT_RA = 5, /* elasticRA3 */
+    TCT_RH = 6  /* elasticRH3 */
+};
+
+enum tct_prec { TCT_F32 = 0, TCT_F64 = 1 };
+
+/* Launch shape of the farfield N-body kernel (chosen by the autotuner).
+ *   block:        threads per workgroup (64, 128 or 256);
+ *   obs_per_lane: observation points held per lane (1, 2 or 4);
+ *   src_chunks:   the source points are split into this many contiguous
+ *                 chunks, one grid row each; partial sums are written per
+ *                 chunk and summed in a fixed order by the reduce/scatter
+ *                 step, so results are bit-reproducible. */
+typedef struct {
+    int32_t block;
+    int32_t obs_per_lane;
+    int32_t src_chunks;
+    int32_t reserved;
+} tct_tile_cfg;
+
+/* Points per origin group of the FP32 layout (obs and src).  FP32
+ * coordinates are stored relative to an FP64 origin per group of this many
+ * consecutive points. */
+#define TCT_GROUP 64
+/* Floats (FP32) / doubles (FP64) per source record:
+ * [y0 y1 y2 n0 n1 n2 s0 s1 s2 e0 e1 e2]. */
+#define TCT_REC 12
+
+const char *tct_last_error(void);
+int tct_abi_version(void);
+
+/* ---------------- farfield: layout packing (one-time per point set) ----
+ * Replaces the host->device uploads of tectosaur/farfield.py:26-31 and
+ * tectosaur/ops/sparse_farfield_op.py:27-35.
+ *
+ * tct_far_pack_obs: pts/ns [n][3] f64 (dev) ->
+ *   loc32  [6][n_pad] f32: x, y, z relative to the group origin, then the
+ *          unit normal (dev, n_pad = round_up(n, TCT_GROUP));
+ *   org    [n_pad/TCT_GROUP][3] f64 group origins (mean of the group);
+ *   soa64  [6][n_pad] f64 absolute coordinates + normals (for TCT_F64);
+ *   padding points are placed far away with zero normal. */
+int tct_far_pack_obs(int64_t n, const double *pts, const double *ns, int64_t n_pad,
+                     float *loc32, double *org, double *soa64, void *stream);
+
+/* tct_far_pack_src: geometry part of the source records:
+ *   rec32 [n_pad][TCT_REC] f32 (y local to its group origin, unit normal),
+ *   org   [n_pad/TCT_GROUP][3] f64,
+ *   rec64 [n_pad][TCT_REC] f64 (absolute).  Strength fields are zeroed. */
+int tct_far_pack_src(int64_t n, const double *pts, const double *ns, int64_t n_pad,
+                     float *rec32, double *org, double *rec64, void *stream);
+
+/* ---------------- farfield: per-matvec strengths -------------------------
+ * tct_far_strength_pts: point strengths vec [n][3] f64 (the "input" of
+ * farfield_pts{K}, tectosaur/farfield_direct.cl:70-72) -> record strength
+ * and kernel-specific extra fields, for precision prec. */
+int tct_far_strength_pts(int kernel, int prec, int64_t n, const double *vec,
+                         const double *params_host, void *rec, void *stream);
+
+/* tct_far_gather_tris: triangle DOF input v[n_tris*9] f64 -> per-quadrature
+ * point strengths s = J_t w_q sum_b phi_b(q) v[9t+3b+:] (the interpolation
+ * of tectosaur/matrix_free.cl:84-90 with the factor of :83 folded in).
+ *   tri_of_pt [n_pts] int32: DOF-space triangle of each source point
+ *             (points of one triangle are consecutive, nq of them);
+ *   qphi      [nq][3] f64 basis values at the rule's points;
+ *   wj        [n_pts] f64: J_t * w_q per point. */
+int tct_far_gather_tris(int kernel, int prec, int64_t n_pts, int nq, const int32_t *tri_of_pt,
+                        const double *qphi, const double *wj, const double *v,
+                        const double *params_host, void *rec, void *stream);
+
+/* ---------------- farfield: the N-body kernel ---------------------------
+ * Reference: farfield_pts{K} (tectosaur/farfield_direct.cl:15-113), the
+ * hot loop of farfield_tris_to_tris{K} (tectosaur/matrix_free.cl:47-101).
+ * out_partial [cfg.src_chunks][3][n_obs_pad] f64 (dev): per-chunk sums over
+ * the chunk's source points of K(y_j - x_i) s_j, pairs with r2 == 0
+ * skipped.  TCT_F32: FP32 evaluation and FP32 accumulation within each
+ * 64-point source tile, FP64 accumulation across tiles; pairs with
+ * r2 <= eps2 are skipped, where eps2 bounds the rounding of one point stored
+ * relative to two different group origins (see the packing).  TCT_F64: FP64
+ * throughout, absolute coordinates, exact r2 == 0 test (eps2 ignored).
+ * Kernels U, T, A, H. */
+int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float *loc32,
+                const double *obs_org, const double *obs_soa64, int64_t n_src_pad,
+                const void *rec, const double *src_org, const double *params_host,
+                double eps2, const tct_tile_cfg *cfg, double *out_partial, void *stream);
+
+/* Sum of the chunk partials: out [n][3] f64 (point order of the packing). */
+int tct_far_reduce_pts(int64_t n, int64_t n_obs_pad, int chunks, const double *partial,
+                       double *out, void *stream);
+
+/* Galerkin scatter (tectosaur/matrix_free.cl:94-98): out[9t+3b+d] =
+ *   sum_q wj_obs[p] phi_b(q) sum_chunks partial[c][d][p],  p = pt_of_tri[t]*nq + q.
+ * pt_of_tri [n_tris] int32: first-point position (divided by nq) of each
+ * DOF-space obs triangle in the packed order. */
+int tct_far_scatter_tris(int64_t n_tris, int nq, const int32_t *pt_of_tri, const double *qphi,
+                         const double *wj_obs, int64_t n_obs_pad, int chunks,
+                         const double *partial, double *out, void *stream);
+
+/* ---------------- nearfield pair integrator (FP64) ----------------------
+ * Reference: single_pairs{Z,N} / single_pairsN_adj (tectosaur/assemble.cl
+ * :24-103) driven by PairsIntegrator (tectosaur/nearfield/
+ * pairs_integrator.py:27-93).
+ *   qx [nq][4] f64 (obs xhat, obs yhat, src xhat, src yhat), qw [nq] f64;
+ *   pts [n_pts][3] f64, tris [n_tris][3] int32;
+ *   pairs [n_pairs][pair_width] int32, pair_width 2 (obs, src) or 5
+ *         (obs, src, obs_clicks, src_clicks, src_flip) -- width 5 rotates
+ *         and de-rotates exactly as assemble.cl:77-101;
+ *   check0: skip quadrature points with r2 == 0 (the "Z" variant);
+ *   out [n_pairs][81] f64.
+ * lanes_per_pair (1, 4, 16 or 64) sets how many lanes share one pair's
+ * quadrature points; 0 picks from nq. */
+int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, const double *qw,
+                        const double *pts, const int32_t *tris, int64_t n_pairs,
+                        const int32_t *pairs, int pair_width, int check0,
+                        const double *params_host, int lanes_per_pair, double *out,
+                        void *stream);
+
+/* out[i] = a[i] - b[i] over n doubles (nearfield correction, tectosaur/
+ * nearfield/nearfield_op.py:135-141). */
+int tct_sub_f64(int64_t n, const double *a, const double *b, double *out, void *stream);
+
+/* ---------------- block sparse matvec (FP64) -----------------------------
+ * Reference: bsrmv9 / bcoomv9 (tectosaur/util/fast_sparse.cpp:71-163).
+ * y[9r+i] (+)= sum_k A_k[i][j] x[9 col_k + j] over the blocks of block-row r.
+ * indptr [n_block_rows+1] int32, indices [n_blocks] int32,
+ * data [n_blocks][81] f64.  accumulate != 0 adds into y. */
+int tct_bsr9_spmv(int64_t n_block_rows, const int32_t *indptr, const int32_t *indices,
+                  const double *data, const double *x, double *y, int accumulate,
+                  void *stream);
+
+/* CSR SpMV y (+)= A x, FP64 (the constraint matrix cm and cm^T of
+ * tectosaur/constraints.py:10-17 applied each GMRES iteration). */
+int tct_csr_spmv(int64_t n_rows, const int32_t *indptr, const int32_t *indices,
+                 const double *data, const double *x, double *y, int accumulate, void *stream);
+
+/* ---------------- host-side mesh utilities (CPU, OpenMP) -----------------
+ * Reference: fast_find_nearfield.get_nearfield + split_adjacent_close
+ * (tectosaur/mesh/fast_find_nearfield.cpp:22-217).
+ * tct_find_near_pairs: all (i, j) with |c_i - c_j|^2 <= ((r_i + r_j) thr)^2
+ * (c: centroids [n][3], r: bounding radii).  *n_out receives the pair
+ * count; the pairs (sorted by i, then j) are written to out [cap][2] int64
+ * only when cap >= *n_out, otherwise nothing is written and the caller
+ * retries with a larger buffer. */
+int tct_find_near_pairs(int64_t n_a, const double *ca, const double *ra, int64_t n_b,
+                        const double *cb, const double *rb, double threshold, int64_t *out,
+                        int64_t cap, int64_t *n_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TCT_HIP_H */

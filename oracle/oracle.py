@@ -1,0 +1,261 @@
+"""CPU oracle for the farfield / nearfield hot path -- TEST INFRASTRUCTURE.
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py may import this module, and only as the checker.  The product
+package ``tectosaur_amd`` never imports it.
+
+It loads ``liboracle.so`` (built from tct_oracle.c by the Makefile here) and
+adds numpy restatements of the host-side reference steps.
+
+Pinning (what fixes each function independently of this restatement):
+
+  kernel_tensor / kernel_vector   U, T, A, H: sympy derivation of the Kelvin
+      solution and its traction operators (tests/golden/make_kernel_golden.py,
+      the method of tectosaur tests/elastic_derive.py:47-147), plus the
+      reference's own tensor==vector identity.
+  farfield_pts                    brute-force sums of the pinned point
+      kernels; scaling law K(lambda x) = lambda^(scale_type) K(x).
+  pairs_integrate (U/T/A/H)       golden master tests/golden_masters/
+      test_farfield_two_tris.npy (H, nq=3); test_gpu_vert_adjacent.npy
+      (H, vertex-adjacent, nq=3); refinement additivity (a parent coincident
+      integral equals the sum of its 16 child-pair integrals).
+  pairs_integrate (RT/RA/RH)      refinement additivity and convergence in n;
+      parity unpinned against the reference's numbers (no fixture exists and
+      the reference cannot run here).
+  farfield_tris                   equals the dense pair integral of the same
+      rule summed against the input (two independent code paths), plus the
+      golden two-triangle fixture through that identity.
+  near_pairs / split_adjacent     golden masters test_close_or_touching.npy,
+      test_find_nearfield_real.npy, test_find_adjacency.npy and the exact
+      assertions of tectosaur tests/test_find_near_adj.py:18-37.
+  bcoo_matvec                     dense matrix-vector product
+      (tectosaur tests/test_sparse.py:8-73 semantics).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+KERNEL_IDS = {
+    'elasticU3': 0, 'elasticT3': 1, 'elasticA3': 2, 'elasticH3': 3,
+    'elasticRT3': 4, 'elasticRA3': 5, 'elasticRH3': 6,
+}
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, 'liboracle.so')
+        if not os.path.exists(path):
+            raise RuntimeError('oracle not built: run make -C oracle')
+        L = ctypes.CDLL(path)
+        L.or_kernel_tensor.argtypes = [ctypes.c_int, _d, _d, _d, _d, _d]
+        L.or_kernel_vector.argtypes = [ctypes.c_int, _d, _d, _d, _d, _d, _d]
+        L.or_farfield_pts.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, ctypes.c_int64,
+                                      _d, _d, _d, _d, _i64, ctypes.c_int64, _d]
+        L.or_pairs_integrate.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d, _i32,
+                                         ctypes.c_int64, _i64, ctypes.c_int, ctypes.c_int,
+                                         _d, _d]
+        L.or_farfield_tris.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d,
+                                       ctypes.c_int64, _i32, ctypes.c_int64, _i32, _d, _d,
+                                       _i64, ctypes.c_int64, _d]
+        L.or_near_pairs_bruteforce.argtypes = [ctypes.c_int64, _d, _d, ctypes.c_int64, _d, _d,
+                                               ctypes.c_double, _i64, ctypes.c_int64]
+        L.or_near_pairs_bruteforce.restype = ctypes.c_int64
+        L.or_bcoo_matvec.argtypes = [ctypes.c_int64, ctypes.c_int, _i64, _i64, _d, _d, _d]
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def kernel_tensor(K, D, m, n, params):
+    """3x3 kernel K[d_obs, d_src] at displacement D = src - obs."""
+    out = np.zeros(9)
+    D, m, n, params = (_f64(a) for a in (D, m, n, params))
+    lib().or_kernel_tensor(KERNEL_IDS[K], _p(D), _p(m), _p(n), _p(params), _p(out))
+    return out.reshape(3, 3)
+
+
+def kernel_vector(K, D, m, n, vin, params):
+    out = np.zeros(3)
+    D, m, n, vin, params = (_f64(a) for a in (D, m, n, vin, params))
+    lib().or_kernel_vector(KERNEL_IDS[K], _p(D), _p(m), _p(n), _p(vin), _p(params), _p(out))
+    return out
+
+
+def farfield_pts(K, obs_pts, obs_ns, src_pts, src_ns, vec, params, rows=None):
+    """a1: out[i] = sum_j K(y_j - x_i) in_j (r2 == 0 skipped)."""
+    obs_pts, obs_ns, src_pts, src_ns, vec, params = (
+        _f64(a) for a in (obs_pts, obs_ns, src_pts, src_ns, vec, params))
+    n_obs, n_src = obs_pts.shape[0], src_pts.shape[0]
+    if rows is None:
+        out = np.zeros(n_obs * 3)
+        rp, nr = None, 0
+    else:
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        out = np.zeros(rows.shape[0] * 3)
+        rp, nr = _p(rows, _i64), rows.shape[0]
+    lib().or_farfield_pts(KERNEL_IDS[K], n_obs, _p(obs_pts), _p(obs_ns), n_src, _p(src_pts),
+                          _p(src_ns), _p(vec), _p(params), rp, nr, _p(out))
+    return out
+
+
+def pairs_integrate(K, qx, qw, pts, tris, pairs, check0, params):
+    """a5: [n, 3, 3, 3, 3] pair integrals; pairs has 2 or 5 columns."""
+    qx, qw, pts, params = (_f64(a) for a in (qx, qw, pts, params))
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64)
+    n = pairs.shape[0]
+    out = np.zeros(n * 81)
+    if n > 0:
+        lib().or_pairs_integrate(KERNEL_IDS[K], qx.shape[0], _p(qx), _p(qw), _p(pts),
+                                 _p(tris, _i32), n, _p(pairs, _i64), pairs.shape[1],
+                                 int(bool(check0)), _p(params), _p(out))
+    return out.reshape((n, 3, 3, 3, 3))
+
+
+def farfield_tris(K, qx2, qw2, pts, obs_tris, src_tris, vin, params, rows=None):
+    """a2: matrix-free tri-to-tri farfield, out[n_obs * 9]."""
+    qx2, qw2, pts, vin, params = (_f64(a) for a in (qx2, qw2, pts, vin, params))
+    obs_tris = np.ascontiguousarray(obs_tris, dtype=np.int32)
+    src_tris = np.ascontiguousarray(src_tris, dtype=np.int32)
+    n_obs = obs_tris.shape[0]
+    if rows is None:
+        out = np.zeros(n_obs * 9)
+        rp, nr = None, 0
+    else:
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        out = np.zeros(rows.shape[0] * 9)
+        rp, nr = _p(rows, _i64), rows.shape[0]
+    lib().or_farfield_tris(KERNEL_IDS[K], qx2.shape[0], _p(qx2), _p(qw2), _p(pts), n_obs,
+                           _p(obs_tris, _i32), src_tris.shape[0], _p(src_tris, _i32),
+                           _p(vin), _p(params), rp, nr, _p(out))
+    return out
+
+
+def tri_centroids_rs(pts, tris):
+    """Centroid and bounding radius per triangle, the arithmetic of
+    tectosaur/mesh/find_near_adj.py:13-20."""
+    tri_pts = pts[tris]
+    centroid = np.sum(tri_pts, axis=1) / 3.0
+    r = np.sqrt(np.max(np.sum((tri_pts - centroid[:, np.newaxis, :]) ** 2, axis=2), axis=1))
+    return centroid, r
+
+
+def near_pairs(a_pts, a_tris, b_pts, b_tris, threshold):
+    """All (i, j) with |c_i - c_j|^2 <= ((r_i + r_j) thr)^2, brute force,
+    row-major order."""
+    ca, ra = tri_centroids_rs(a_pts, a_tris)
+    cb, rb = tri_centroids_rs(b_pts, b_tris)
+    ca, ra, cb, rb = (_f64(x) for x in (ca, ra, cb, rb))
+    cap = 1 << 16
+    while True:
+        out = np.zeros(2 * cap, dtype=np.int64)
+        k = lib().or_near_pairs_bruteforce(ca.shape[0], _p(ca), _p(ra), cb.shape[0], _p(cb),
+                                           _p(rb), float(threshold), _p(out, _i64), cap)
+        if k <= cap:
+            return out[:2 * k].reshape((k, 2))
+        cap = k
+
+
+def split_adjacent_close(close_pairs, tris_a, tris_b):
+    """Classify pairs by shared vertices, tectosaur/mesh/fast_find_nearfield.cpp
+    :178-217: returns (near [n,2], vertex-adj [n,4], edge-adj [n,6]);
+    coincident pairs (3 shared) are dropped."""
+    near, va, ea = [], [], []
+    for i1, i2 in close_pairs:
+        p1 = p2 = None
+        co = False
+        for d1 in range(3):
+            for d2 in range(3):
+                if tris_a[i1, d1] != tris_b[i2, d2]:
+                    continue
+                if p1 is None:
+                    p1 = (d1, d2)
+                elif p2 is None:
+                    p2 = (d1, d2)
+                else:
+                    co = True
+        if co:
+            continue
+        if p1 is None:
+            near.append((i1, i2))
+        elif p2 is None:
+            va.append((i1, i2, p1[0], p1[1]))
+        else:
+            ea.append((i1, i2, p1[0], p1[1], p2[0], p2[1]))
+    return (np.array(near, dtype=np.int64).reshape((-1, 2)),
+            np.array(va, dtype=np.int64).reshape((-1, 4)),
+            np.array(ea, dtype=np.int64).reshape((-1, 6)))
+
+
+def edge_adj_orient(touching_verts):
+    """tectosaur/nearfield/nearfield_op.py:44-50."""
+    tv = sorted(touching_verts)
+    if tv[0] == 0:
+        return 2 if tv[1] == 2 else 0
+    return 1
+
+
+def resolve_ea_rotation(tris, ea):
+    """tectosaur/nearfield/nearfield_op.py:52-62: (obs, src, obs_clicks,
+    src_clicks, src_flip) per edge-adjacent pair."""
+    out = []
+    for row in ea:
+        oc = edge_adj_orient([row[2], row[4]])
+        sc = edge_adj_orient([row[3], row[5]])
+        flip = (tris[row[0], (0 + oc) % 3] != tris[row[1], (1 + sc) % 3] or
+                tris[row[0], (1 + oc) % 3] != tris[row[1], (0 + sc) % 3])
+        out.append((row[0], row[1], oc, sc, int(flip)))
+    return np.array(out, dtype=np.int64).reshape((-1, 5))
+
+
+def bcoo_matvec(rows, cols, data, x, shape):
+    rows = np.ascontiguousarray(rows, dtype=np.int64)
+    cols = np.ascontiguousarray(cols, dtype=np.int64)
+    data = _f64(data)
+    x = _f64(x)
+    bs = data.shape[1]
+    y = np.zeros(shape[0])
+    lib().or_bcoo_matvec(rows.shape[0], bs, _p(rows, _i64), _p(cols, _i64), _p(data), _p(x),
+                         _p(y))
+    return y
+
+
+def nearfield_blocks(pts, tris, nq_co, nq_ea, nq_va, nq_far, nq_near, near_threshold,
+                     K_near, K_far, params, rules):
+    """The nearfield assembly of tectosaur/nearfield/nearfield_op.py:95-141
+    for obs_subset == src_subset == all triangles, computed with the oracle
+    pair integrator.  ``rules`` supplies the quadrature rules (a dict with
+    'co', 'ea', 'va', 'near', 'far' -> (qx, qw)) so this module does not
+    depend on the product package.  Returns a dict class -> (pairs, corrected
+    blocks [n,9,9], uncorrected blocks [n,9,9])."""
+    n = tris.shape[0]
+    co = np.stack([np.arange(n), np.arange(n)], axis=1)
+    close = near_pairs(pts, tris, pts, tris, near_threshold)
+    near, va4, ea6 = split_adjacent_close(close, tris, tris)
+    va = np.concatenate([va4, np.zeros((va4.shape[0], 1), dtype=np.int64)], axis=1)
+    ea = resolve_ea_rotation(tris, ea6)
+    out = {}
+    for name, prs, q, check0 in (('co', co, rules['co'], True), ('ea', ea, rules['ea'], True),
+                                 ('va', va, rules['va'], True),
+                                 ('near', near, rules['near'], False)):
+        acc = pairs_integrate(K_near, q[0], q[1], pts, tris, prs, check0, params)
+        corr = pairs_integrate(K_far, rules['far'][0], rules['far'][1], pts, tris,
+                               prs[:, :2], name == 'co', params)
+        out[name] = (prs, (acc - corr).reshape((-1, 9, 9)), acc.reshape((-1, 9, 9)))
+    return out

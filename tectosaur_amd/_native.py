@@ -1,0 +1,113 @@
+"""ctypes binding of libtct_hip.so (the C ABI declared in include/tct_hip.h).
+
+torch is imported before the library is opened: torch ships its own HIP
+runtime whose soname (libamdhip64.so.7) is the one libtct_hip.so links
+against, so the dynamic loader binds both to the same runtime and torch's
+device pointers and streams are valid here.
+
+There is no fallback: if the library is missing, every call raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must be loaded before the HIP library)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libtct_hip.so')
+
+KERNEL_IDS = {
+    'elasticU3': 0, 'elasticT3': 1, 'elasticA3': 2, 'elasticH3': 3,
+    'elasticRT3': 4, 'elasticRA3': 5, 'elasticRH3': 6,
+}
+F32, F64 = 0, 1
+GROUP = 64
+REC = 12
+
+
+class TctError(RuntimeError):
+    pass
+
+
+class TileCfg(ctypes.Structure):
+    _fields_ = [('block', ctypes.c_int32), ('obs_per_lane', ctypes.c_int32),
+                ('src_chunks', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    'tct_last_error': [],
+    'tct_abi_version': [],
+    'tct_far_pack_obs': [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    'tct_far_pack_src': [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    'tct_far_strength_pts': [_int, _int, _i64, _vp, _vp, _vp, _vp],
+    'tct_far_gather_tris': [_int, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    'tct_far_p2p': [_int, _int, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp],
+    'tct_far_reduce_pts': [_i64, _i64, _int, _vp, _vp, _vp],
+    'tct_far_scatter_tris': [_i64, _int, _vp, _vp, _vp, _i64, _int, _vp, _vp, _vp],
+    'tct_pairs_integrate': [_int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _vp, _int,
+                            _vp, _vp],
+    'tct_sub_f64': [_i64, _vp, _vp, _vp, _vp],
+    'tct_bsr9_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
+    'tct_csr_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
+    'tct_find_near_pairs': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _vp, _i64, _vp],
+}
+_RESTYPE = {'tct_last_error': ctypes.c_char_p}
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises TctError if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise TctError('libtct_hip.so not built (run python -m tectosaur_amd.build)')
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, ctypes.c_int)
+        _LIB = L
+    return _LIB
+
+
+def call(name, *args):
+    """Call an entry point and raise TctError with tct_last_error() on failure."""
+    L = lib()
+    rc = getattr(L, name)(*args)
+    if rc != 0:
+        msg = L.tct_last_error().decode(errors='replace')
+        raise TctError('%s failed (%d): %s' % (name, rc, msg))
+    return rc
+
+
+def ptr(t):
+    """Device (or host) address of a tensor / numpy array, or None."""
+    if t is None:
+        return None
+    if isinstance(t, torch.Tensor):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def require_device():
+    """The HIP path only: raise when no GPU is visible."""
+    if not torch.cuda.is_available():
+        raise TctError('no HIP device visible: tectosaur_amd runs on the GPU only')
+    lib()
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def params_array(params):
+    import numpy as np
+    return np.ascontiguousarray(np.array(params, dtype=np.float64))

@@ -1,0 +1,112 @@
+"""Launch-shape autotuner for the farfield N-body kernel.
+
+The tunable shape is ``TileCfg(block, obs_per_lane, src_chunks)``.  A tuning
+run times every candidate on the device for one (kernel, precision, size
+class) and records the fastest in a JSON table (``autotune_table.json``
+next to this file, or the path in TCT_AUTOTUNE_TABLE).  ``lookup`` reads the
+table and falls back to a heuristic shape when no entry matches.
+
+Size classes are powers of two of the padded observation and source counts,
+so one entry serves a range of problem sizes.
+"""
+import json
+import math
+import os
+import threading
+import time
+
+import torch
+
+from . import _native as N
+
+_LOCK = threading.Lock()
+_TABLE = None
+DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'autotune_table.json')
+
+
+def table_path():
+    return os.environ.get('TCT_AUTOTUNE_TABLE', DEFAULT_PATH)
+
+
+def _size_class(n):
+    return int(math.ceil(math.log2(max(n, 1))))
+
+
+def _key(K_name, prec, n_obs_pad, n_src_pad, device):
+    arch = 'gfx950'
+    if device is not None and torch.cuda.is_available():
+        arch = torch.cuda.get_device_properties(device).gcnArchName.split(':')[0]
+    return '%s|%s|%s|o%d|s%d' % (arch, K_name, prec, _size_class(n_obs_pad),
+                                 _size_class(n_src_pad))
+
+
+def _load():
+    global _TABLE
+    with _LOCK:
+        if _TABLE is None:
+            try:
+                with open(table_path()) as f:
+                    _TABLE = json.load(f)
+            except (OSError, ValueError):
+                _TABLE = {}
+        return _TABLE
+
+
+def lookup(K_name, prec, n_obs_pad, n_src_pad, device=None):
+    from .farfield import default_tile_cfg, GROUP
+    t = _load()
+    e = t.get(_key(K_name, prec, n_obs_pad, n_src_pad, device))
+    if e is not None:
+        n_tiles = n_src_pad // GROUP
+        chunks = max(1, min(int(e['src_chunks']), n_tiles))
+        return N.TileCfg(int(e['block']), int(e['obs_per_lane']), chunks, 0)
+    return default_tile_cfg(n_obs_pad, n_src_pad, prec, device)
+
+
+def candidates(n_obs_pad, n_src_pad, n_cu=256):
+    from .farfield import GROUP
+    n_tiles = n_src_pad // GROUP
+    out = []
+    for block in (64, 128, 256):
+        for per_lane in (1, 2, 4):
+            obs_blocks = -(-n_obs_pad // (block * per_lane))
+            for waves_target in (4, 8, 16, 32):
+                chunks = max(1, min(n_tiles, -(-waves_target * n_cu // obs_blocks)))
+                cfg = (block, per_lane, chunks)
+                if cfg not in out:
+                    out.append(cfg)
+    return out
+
+
+def tune(engine, prec, reps=3, save=True, verbose=False):
+    """Time every candidate launch shape on ``engine`` (a FarfieldEngine with
+    strengths already set) and store the fastest.  Returns (cfg, seconds)."""
+    n_cu = torch.cuda.get_device_properties(engine.device).multi_processor_count
+    best = None
+    for block, per_lane, chunks in candidates(engine.n_obs_pad, engine.n_src_pad, n_cu):
+        cfg = N.TileCfg(block, per_lane, chunks, 0)
+        engine.run(prec, cfg)  # warm-up / allocation
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            engine.run(prec, cfg)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        if verbose:
+            print('tune %s %s block=%d P=%d chunks=%d: %.3f ms' % (
+                engine.K.name, prec, block, per_lane, chunks, dt * 1e3), flush=True)
+        if best is None or dt < best[1]:
+            best = (cfg, dt)
+    engine.tile_cfg[prec] = best[0]
+    if save:
+        t = _load()
+        with _LOCK:
+            t[_key(engine.K.name, prec, engine.n_obs_pad, engine.n_src_pad, engine.device)] = {
+                'block': best[0].block, 'obs_per_lane': best[0].obs_per_lane,
+                'src_chunks': best[0].src_chunks, 'seconds': best[1]}
+            try:
+                with open(table_path(), 'w') as f:
+                    json.dump(t, f, indent=1, sort_keys=True)
+            except OSError:
+                pass
+    return best

@@ -1,0 +1,602 @@
+// Farfield direct N-body for the elastic Kelvin kernels U, T, A, H on gfx950.
+//
+// Reference semantics: farfield_pts{K} (tectosaur/farfield_direct.cl:15-113)
+// and the quadrature-point loop of farfield_tris_to_tris{K}
+// (tectosaur/matrix_free.cl:47-101): out_i = sum_j K(y_j - x_i) s_j with
+// pairs at r2 == 0 skipped.
+//
+// Design (MI355X / CDNA4, wave64):
+//  * One lane owns P observation points (coalesced SoA loads, kept in VGPRs).
+//  * Source points are wave-uniform: each wave walks the same source list,
+//    so the 12-word source record is fetched with scalar loads into SGPRs
+//    and feeds every lane's FMAs directly -- no LDS staging and no barriers.
+//  * The kernel body is the algebraically factored form of the reference's
+//    vector code (source strengths pre-scaled by the kernel constant, per-
+//    source dot products precomputed in the record), pure VALU, one
+//    v_rsq_f32 per pair.  It is not a dense contraction, so no MFMA.
+//  * FP32 path: coordinates are stored relative to an FP64 origin per group
+//    of 64 consecutive points.  Per 64-point source tile the lane forms
+//    x' = x_local - (O_src - O_obs) once, so D = y_local - x' is accurate to
+//    FP32 relative to |D| + the tile extent rather than to the mesh extent.
+//    Accumulation is FP32 within a source tile and FP64 across tiles.
+//  * The source list can be split into chunks along grid.y; each chunk
+//    writes its own partial sum and the reduce/scatter kernels add the
+//    chunks in a fixed order (deterministic results).
+#include "tct_common.h"
+
+namespace tct {
+
+constexpr int G = TCT_GROUP;  // points per origin group / source tile
+constexpr int R = TCT_REC;    // words per source record
+
+// Kernel constants after folding the leading constant into the strengths.
+// eps2: pairs with r2 <= eps2 are skipped.  The FP64 path uses 0 (the
+// reference's exact r2 == 0 test).  The FP32 path uses a bound on the
+// rounding of a point stored relative to two different group origins, so a
+// point paired with itself is skipped even when obs and src are packed
+// differently; real point spacings are many orders of magnitude larger.
+template <typename T>
+struct KConst {
+    T c0, c1, c2, c3, c4, eps2;
+};
+
+__device__ __forceinline__ float rsq_masked(float r2, float eps2) {
+    float v = __builtin_amdgcn_rsqf(r2);
+    return r2 > eps2 ? v : 0.0f;
+}
+__device__ __forceinline__ double rsq_masked(double r2, double eps2) {
+    double v = rsqrt(r2);
+    return r2 > eps2 ? v : 0.0;
+}
+
+// One (observation point, source point) pair, accumulate into acc[3].
+// x: observation point (already shifted into the source tile frame);
+// m: observation unit normal; y, n, s: source point, normal, scaled strength;
+// e0, e1: per-source precomputed terms.
+template <int KID, typename T>
+__device__ __forceinline__ void pair_eval(const T x[3], const T m[3], T y0, T y1, T y2, T n0,
+                                          T n1, T n2, T s0, T s1, T s2, T e0, T e1,
+                                          const KConst<T> &k, T acc[3]) {
+    const T D0 = y0 - x[0], D1 = y1 - x[1], D2 = y2 - x[2];
+    const T r2 = D0 * D0 + D1 * D1 + D2 * D2;
+    const T invr = rsq_masked(r2, k.eps2);
+    const T w = invr * invr;
+    if constexpr (KID == TCT_U) {
+        // U: (3-4nu)/r s + (D.s) D / r^3, strengths scaled by CsU1.
+        const T ds = D0 * s0 + D1 * s1 + D2 * s2;
+        const T a = k.c0 * invr;
+        const T b = invr * w * ds;
+        acc[0] += a * s0 + b * D0;
+        acc[1] += a * s1 + b * D1;
+        acc[2] += a * s2 + b * D2;
+    } else if constexpr (KID == TCT_T) {
+        // T: A(-(n.D) s + (n x D) x s) - C (n.D)(D.s) D with
+        // (n x D) x s = D (n.s) - n (D.s); strengths scaled by CsT1,
+        // k.c0 = CsT0 / CsT1, e0 = n.s.
+        const T q3 = invr * w;
+        const T al = k.c0 * q3;
+        const T rn = n0 * D0 + n1 * D1 + n2 * D2;
+        const T ds = D0 * s0 + D1 * s1 + D2 * s2;
+        const T be = -al * rn;
+        const T ga = -al * ds;
+        const T de = al * e0 - q3 * w * rn * ds;
+        acc[0] += be * s0 + ga * n0 + de * D0;
+        acc[1] += be * s1 + ga * n1 + de * D1;
+        acc[2] += be * s2 + ga * n2 + de * D2;
+    } else if constexpr (KID == TCT_A) {
+        // A: the adjoint form with the observation normal m.
+        const T q3 = invr * w;
+        const T al = k.c0 * q3;
+        const T rn = m[0] * D0 + m[1] * D1 + m[2] * D2;
+        const T ds = D0 * s0 + D1 * s1 + D2 * s2;
+        const T sm = m[0] * s0 + m[1] * s1 + m[2] * s2;
+        const T be = al * rn;
+        const T ga = -al * ds;
+        const T de = al * sm + q3 * w * rn * ds;
+        acc[0] += be * s0 + ga * m[0] + de * D0;
+        acc[1] += be * s1 + ga * m[1] + de * D1;
+        acc[2] += be * s2 + ga * m[2] + de * D2;
+    } else {
+        // H with strengths scaled by CsH0; k = (CsH1, CsH3, 3 CsH1, nu, 3 nu),
+        // e0 = CsH2 (n.s), e1 = 3 CsH1 (n.s).
+        const T Q = invr * w;
+        const T a = n0 * D0 + n1 * D1 + n2 * D2;
+        const T b = m[0] * D0 + m[1] * D1 + m[2] * D2;
+        const T c = s0 * D0 + s1 * D1 + s2 * D2;
+        const T sm = s0 * m[0] + s1 * m[1] + s2 * m[2];
+        const T mn = m[0] * n0 + m[1] * n1 + m[2] * n2;
+        const T wa = w * a;
+        const T wbc = w * b * c;
+        const T MT = Q * (e0 + k.c2 * wa * c);
+        const T NT = Q * (k.c0 * sm + k.c1 * wbc);
+        const T inner = k.c3 * sm - T(5) * wbc;
+        const T DT = Q * w * (e1 * b + k.c1 * c * mn + T(3) * a * inner);
+        const T ST = Q * (k.c4 * wa * b + k.c0 * mn);
+        acc[0] += n0 * NT + m[0] * MT + D0 * DT + s0 * ST;
+        acc[1] += n1 * NT + m[1] * MT + D1 * DT + s1 * ST;
+        acc[2] += n2 * NT + m[2] * MT + D2 * DT + s2 * ST;
+    }
+}
+
+template <int KID>
+constexpr bool needs_obs_normal() { return KID == TCT_A || KID == TCT_H; }
+
+// ---------------------------------------------------------------------------
+// FP32 N-body.  grid = (ceil(n_obs_pad / (blockDim*P)), chunks).
+// loc32 [6][n_obs_pad]; obs_org [n_obs_pad/G][3]; rec [n_src_pad][R];
+// src_org [n_src_pad/G][3]; partial [chunks][3][n_obs_pad].
+template <int KID, int P>
+__global__ __launch_bounds__(256) void far_p2p_f32_kernel(
+    int n_obs_pad, const float *__restrict__ loc32, const double *__restrict__ obs_org,
+    int n_tiles, const float *__restrict__ rec, const double *__restrict__ src_org,
+    KConst<float> k, double *__restrict__ partial) {
+    const int bd = blockDim.x;
+    const int base = blockIdx.x * bd * P;
+    const int chunks = gridDim.y;
+    const int c = blockIdx.y;
+    const int t0 = (int)(((long)n_tiles * c) / chunks);
+    const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
+
+    float xl[P][3], m[P][3];
+    double oo[P][3], acc64[P][3];
+    int idx[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        int i = base + p * bd + (int)threadIdx.x;
+        idx[p] = i;
+        int ic = i < n_obs_pad ? i : n_obs_pad - 1;
+        int g = ic / G;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            xl[p][d] = loc32[(size_t)d * n_obs_pad + ic];
+            m[p][d] = needs_obs_normal<KID>() ? loc32[(size_t)(3 + d) * n_obs_pad + ic] : 0.0f;
+            oo[p][d] = obs_org[3 * g + d];
+            acc64[p][d] = 0.0;
+        }
+    }
+
+    for (int t = t0; t < t1; ++t) {
+        const double os0 = src_org[3 * t], os1 = src_org[3 * t + 1], os2 = src_org[3 * t + 2];
+        float xs[P][3], acc[P][3];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            xs[p][0] = xl[p][0] - (float)(os0 - oo[p][0]);
+            xs[p][1] = xl[p][1] - (float)(os1 - oo[p][1]);
+            xs[p][2] = xl[p][2] - (float)(os2 - oo[p][2]);
+            acc[p][0] = acc[p][1] = acc[p][2] = 0.0f;
+        }
+        const float *r = rec + (size_t)t * G * R;
+#pragma unroll 4
+        for (int j = 0; j < G; ++j) {
+            const float *q = r + j * R;
+            const float y0 = q[0], y1 = q[1], y2 = q[2];
+            const float n0 = q[3], n1 = q[4], n2 = q[5];
+            const float s0 = q[6], s1 = q[7], s2 = q[8];
+            const float e0 = q[9], e1 = q[10];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+                pair_eval<KID, float>(xs[p], m[p], y0, y1, y2, n0, n1, n2, s0, s1, s2, e0, e1, k,
+                                      acc[p]);
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            acc64[p][0] += (double)acc[p][0];
+            acc64[p][1] += (double)acc[p][1];
+            acc64[p][2] += (double)acc[p][2];
+        }
+    }
+
+    double *out = partial + (size_t)c * 3 * n_obs_pad;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (idx[p] < n_obs_pad) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) out[(size_t)d * n_obs_pad + idx[p]] = acc64[p][d];
+        }
+    }
+}
+
+// FP64 N-body ("plain FP64" path): absolute coordinates, FP64 throughout.
+// soa64 [6][n_obs_pad]; rec64 [n_src_pad][R].
+template <int KID, int P>
+__global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
+                                                          const double *__restrict__ soa64,
+                                                          int n_tiles,
+                                                          const double *__restrict__ rec,
+                                                          KConst<double> k,
+                                                          double *__restrict__ partial) {
+    const int bd = blockDim.x;
+    const int base = blockIdx.x * bd * P;
+    const int chunks = gridDim.y;
+    const int c = blockIdx.y;
+    const int t0 = (int)(((long)n_tiles * c) / chunks);
+    const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
+
+    double x[P][3], m[P][3], acc[P][3];
+    int idx[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        int i = base + p * bd + (int)threadIdx.x;
+        idx[p] = i;
+        int ic = i < n_obs_pad ? i : n_obs_pad - 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            x[p][d] = soa64[(size_t)d * n_obs_pad + ic];
+            m[p][d] = needs_obs_normal<KID>() ? soa64[(size_t)(3 + d) * n_obs_pad + ic] : 0.0;
+            acc[p][d] = 0.0;
+        }
+    }
+    const double *r = rec + (size_t)t0 * G * R;
+    const int nj = (t1 - t0) * G;
+#pragma unroll 2
+    for (int j = 0; j < nj; ++j) {
+        const double *q = r + (size_t)j * R;
+        const double y0 = q[0], y1 = q[1], y2 = q[2];
+        const double n0 = q[3], n1 = q[4], n2 = q[5];
+        const double s0 = q[6], s1 = q[7], s2 = q[8];
+        const double e0 = q[9], e1 = q[10];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+            pair_eval<KID, double>(x[p], m[p], y0, y1, y2, n0, n1, n2, s0, s1, s2, e0, e1, k,
+                                   acc[p]);
+    }
+    double *out = partial + (size_t)c * 3 * n_obs_pad;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (idx[p] < n_obs_pad) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) out[(size_t)d * n_obs_pad + idx[p]] = acc[p][d];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Layout packing.  One wave per 64-point group.
+constexpr double FAR_AWAY = 1.0e12;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void pack_obs_kernel(int64_t n, const double *__restrict__ pts,
+                                const double *__restrict__ ns, int64_t n_pad,
+                                float *__restrict__ loc32, double *__restrict__ org,
+                                double *__restrict__ soa64) {
+    const int64_t g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t i = g * G + lane;
+    const bool valid = i < n;
+    double p[3], nn[3];
+    for (int d = 0; d < 3; ++d) {
+        p[d] = valid ? pts[3 * i + d] : 0.0;
+        nn[d] = valid ? ns[3 * i + d] : 0.0;
+    }
+    const double cnt = wave_sum(valid ? 1.0 : 0.0);
+    double o[3];
+    for (int d = 0; d < 3; ++d) o[d] = wave_sum(p[d]) / cnt;
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) org[3 * g + d] = o[d];
+    for (int d = 0; d < 3; ++d) {
+        loc32[d * n_pad + i] = valid ? (float)(p[d] - o[d]) : 0.0f;
+        loc32[(3 + d) * n_pad + i] = (float)nn[d];
+        soa64[d * n_pad + i] = p[d];
+        soa64[(3 + d) * n_pad + i] = nn[d];
+    }
+}
+
+__global__ void pack_src_kernel(int64_t n, const double *__restrict__ pts,
+                                const double *__restrict__ ns, int64_t n_pad,
+                                float *__restrict__ rec32, double *__restrict__ org,
+                                double *__restrict__ rec64) {
+    const int64_t g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t i = g * G + lane;
+    const bool valid = i < n;
+    double p[3], nn[3];
+    for (int d = 0; d < 3; ++d) {
+        p[d] = valid ? pts[3 * i + d] : 0.0;
+        nn[d] = valid ? ns[3 * i + d] : 0.0;
+    }
+    const double cnt = wave_sum(valid ? 1.0 : 0.0);
+    double o[3];
+    for (int d = 0; d < 3; ++d) o[d] = wave_sum(p[d]) / cnt;
+    if (lane == 0)
+        for (int d = 0; d < 3; ++d) org[3 * g + d] = o[d];
+    float *r32 = rec32 + i * R;
+    double *r64 = rec64 + i * R;
+    for (int d = 0; d < 3; ++d) {
+        r32[d] = valid ? (float)(p[d] - o[d]) : (float)FAR_AWAY;
+        r64[d] = valid ? p[d] : FAR_AWAY;
+        r32[3 + d] = (float)nn[d];
+        r64[3 + d] = nn[d];
+    }
+    for (int d = 6; d < R; ++d) {
+        r32[d] = 0.0f;
+        r64[d] = 0.0;
+    }
+}
+
+// Strength scale and per-source extras for each kernel.
+struct StrengthConsts {
+    double scale;     // multiplies the raw strength
+    double x0, x1;    // e0 = x0 (n.s'), e1 = x1 (n.s')
+};
+
+inline StrengthConsts strength_consts(int kid, const double *params) {
+    ElasticConsts c = elastic_consts(params);
+    StrengthConsts s{1.0, 0.0, 0.0};
+    switch (kid) {
+    case TCT_U: s.scale = c.CsU1; break;
+    case TCT_T: s.scale = c.CsT1; s.x0 = 1.0; break;
+    case TCT_A: s.scale = c.CsT1; break;
+    case TCT_H: s.scale = c.CsH0; s.x0 = c.CsH2; s.x1 = 3.0 * c.CsH1; break;
+    }
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ void write_strength(T *r, double s0, double s1, double s2,
+                                               const StrengthConsts &sc) {
+    const double n0 = (double)r[3], n1 = (double)r[4], n2 = (double)r[5];
+    const double sn = n0 * s0 + n1 * s1 + n2 * s2;
+    r[6] = (T)s0;
+    r[7] = (T)s1;
+    r[8] = (T)s2;
+    r[9] = (T)(sc.x0 * sn);
+    r[10] = (T)(sc.x1 * sn);
+}
+
+template <typename T>
+__global__ void strength_pts_kernel(int64_t n, int64_t n_pad, const double *__restrict__ vec,
+                                    StrengthConsts sc, T *__restrict__ rec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pad) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i < n)
+        for (int d = 0; d < 3; ++d) s[d] = sc.scale * vec[3 * i + d];
+    write_strength<T>(rec + i * R, s[0], s[1], s[2], sc);
+}
+
+template <typename T>
+__global__ void gather_tris_kernel(int64_t n_pts, int64_t n_pad, int nq,
+                                   const int32_t *__restrict__ tri_of_pt,
+                                   const double *__restrict__ qphi, const double *__restrict__ wj,
+                                   const double *__restrict__ v, StrengthConsts sc,
+                                   T *__restrict__ rec) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pad) return;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i < n_pts) {
+        const int64_t t = tri_of_pt[i];
+        const int q = (int)(i % nq);
+        const double f = sc.scale * wj[i];
+        const double *vt = v + 9 * t;
+        const double p0 = qphi[3 * q], p1 = qphi[3 * q + 1], p2 = qphi[3 * q + 2];
+        for (int d = 0; d < 3; ++d) s[d] = f * (p0 * vt[d] + p1 * vt[3 + d] + p2 * vt[6 + d]);
+    }
+    write_strength<T>(rec + i * R, s[0], s[1], s[2], sc);
+}
+
+__global__ void reduce_pts_kernel(int64_t n, int64_t n_obs_pad, int chunks,
+                                  const double *__restrict__ partial, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int d = 0; d < 3; ++d) {
+        double s = 0.0;
+        for (int c = 0; c < chunks; ++c) s += partial[((size_t)c * 3 + d) * n_obs_pad + i];
+        out[3 * i + d] = s;
+    }
+}
+
+__global__ void scatter_tris_kernel(int64_t n_tris, int nq, const int32_t *__restrict__ pt_of_tri,
+                                    const double *__restrict__ qphi,
+                                    const double *__restrict__ wj_obs, int64_t n_obs_pad,
+                                    int chunks, const double *__restrict__ partial,
+                                    double *__restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n_tris * 9) return;
+    const int64_t t = gid / 9;
+    const int bd = (int)(gid % 9);
+    const int b = bd / 3, d = bd % 3;
+    const int64_t p0 = (int64_t)pt_of_tri[t] * nq;
+    double s = 0.0;
+    for (int q = 0; q < nq; ++q) {
+        const int64_t p = p0 + q;
+        double r = 0.0;
+        for (int c = 0; c < chunks; ++c) r += partial[((size_t)c * 3 + d) * n_obs_pad + p];
+        s += wj_obs[p] * qphi[3 * q + b] * r;
+    }
+    out[gid] = s;
+}
+
+// ---------------------------------------------------------------------------
+template <int KID, int P>
+static void launch_f32(dim3 grid, int block, hipStream_t st, int n_obs_pad, const float *loc32,
+                       const double *obs_org, int n_tiles, const float *rec,
+                       const double *src_org, KConst<float> k, double *partial) {
+    hipLaunchKernelGGL((far_p2p_f32_kernel<KID, P>), grid, dim3(block), 0, st, n_obs_pad, loc32,
+                       obs_org, n_tiles, rec, src_org, k, partial);
+}
+
+template <int KID, int P>
+static void launch_f64(dim3 grid, int block, hipStream_t st, int n_obs_pad, const double *soa64,
+                       int n_tiles, const double *rec, KConst<double> k, double *partial) {
+    hipLaunchKernelGGL((far_p2p_f64_kernel<KID, P>), grid, dim3(block), 0, st, n_obs_pad, soa64,
+                       n_tiles, rec, k, partial);
+}
+
+template <typename T>
+static KConst<T> kernel_consts(int kid, const double *params, double eps2) {
+    ElasticConsts c = elastic_consts(params);
+    KConst<T> k{};
+    k.eps2 = (T)eps2;
+    switch (kid) {
+    case TCT_U: k.c0 = (T)(c.CsU0 / c.CsU1); break;
+    case TCT_T:
+    case TCT_A: k.c0 = (T)(c.CsT0 / c.CsT1); break;
+    case TCT_H:
+        k.c0 = (T)c.CsH1;
+        k.c1 = (T)c.CsH3;
+        k.c2 = (T)(3.0 * c.CsH1);
+        k.c3 = (T)c.nu;
+        k.c4 = (T)(3.0 * c.nu);
+        break;
+    }
+    return k;
+}
+
+}  // namespace tct
+
+using namespace tct;
+
+extern "C" int tct_far_pack_obs(int64_t n, const double *pts, const double *ns, int64_t n_pad,
+                                float *loc32, double *org, double *soa64, void *stream) {
+    TCT_CHECK_ARG(n > 0 && n_pad >= n && n_pad % G == 0, "pack_obs: bad sizes n=%lld n_pad=%lld",
+                  (long long)n, (long long)n_pad);
+    TCT_CHECK_ARG(pts && ns && loc32 && org && soa64, "pack_obs: null pointer");
+    hipLaunchKernelGGL(pack_obs_kernel, dim3((unsigned)(n_pad / G)), dim3(G), 0,
+                       as_stream(stream), n, pts, ns, n_pad, loc32, org, soa64);
+    TCT_CHECK_LAUNCH("pack_obs");
+    return TCT_OK;
+}
+
+extern "C" int tct_far_pack_src(int64_t n, const double *pts, const double *ns, int64_t n_pad,
+                                float *rec32, double *org, double *rec64, void *stream) {
+    TCT_CHECK_ARG(n > 0 && n_pad >= n && n_pad % G == 0, "pack_src: bad sizes n=%lld n_pad=%lld",
+                  (long long)n, (long long)n_pad);
+    TCT_CHECK_ARG(pts && ns && rec32 && org && rec64, "pack_src: null pointer");
+    hipLaunchKernelGGL(pack_src_kernel, dim3((unsigned)(n_pad / G)), dim3(G), 0,
+                       as_stream(stream), n, pts, ns, n_pad, rec32, org, rec64);
+    TCT_CHECK_LAUNCH("pack_src");
+    return TCT_OK;
+}
+
+static int check_far_kernel(int kernel) {
+    if (kernel < TCT_U || kernel > TCT_H)
+        return set_error(TCT_EUNSUPPORTED, "farfield point kernel id %d not in U/T/A/H", kernel);
+    return TCT_OK;
+}
+
+extern "C" int tct_far_strength_pts(int kernel, int prec, int64_t n, const double *vec,
+                                    const double *params_host, void *rec, void *stream) {
+    if (int e = check_far_kernel(kernel)) return e;
+    TCT_CHECK_ARG(n >= 0 && vec && rec && params_host, "strength_pts: bad arguments");
+    const int64_t n_pad = ceil_div(n, G) * G;
+    StrengthConsts sc = strength_consts(kernel, params_host);
+    const unsigned nb = (unsigned)ceil_div(n_pad, 256);
+    if (nb == 0) return TCT_OK;
+    if (prec == TCT_F32)
+        hipLaunchKernelGGL(strength_pts_kernel<float>, dim3(nb), dim3(256), 0, as_stream(stream),
+                           n, n_pad, vec, sc, (float *)rec);
+    else
+        hipLaunchKernelGGL(strength_pts_kernel<double>, dim3(nb), dim3(256), 0,
+                           as_stream(stream), n, n_pad, vec, sc, (double *)rec);
+    TCT_CHECK_LAUNCH("strength_pts");
+    return TCT_OK;
+}
+
+extern "C" int tct_far_gather_tris(int kernel, int prec, int64_t n_pts, int nq,
+                                   const int32_t *tri_of_pt, const double *qphi, const double *wj,
+                                   const double *v, const double *params_host, void *rec,
+                                   void *stream) {
+    if (int e = check_far_kernel(kernel)) return e;
+    TCT_CHECK_ARG(n_pts >= 0 && nq > 0 && n_pts % nq == 0, "gather_tris: n_pts %% nq != 0");
+    TCT_CHECK_ARG(tri_of_pt && qphi && wj && v && rec && params_host, "gather_tris: null ptr");
+    const int64_t n_pad = ceil_div(n_pts, G) * G;
+    StrengthConsts sc = strength_consts(kernel, params_host);
+    const unsigned nb = (unsigned)ceil_div(n_pad, 256);
+    if (nb == 0) return TCT_OK;
+    if (prec == TCT_F32)
+        hipLaunchKernelGGL(gather_tris_kernel<float>, dim3(nb), dim3(256), 0, as_stream(stream),
+                           n_pts, n_pad, nq, tri_of_pt, qphi, wj, v, sc, (float *)rec);
+    else
+        hipLaunchKernelGGL(gather_tris_kernel<double>, dim3(nb), dim3(256), 0, as_stream(stream),
+                           n_pts, n_pad, nq, tri_of_pt, qphi, wj, v, sc, (double *)rec);
+    TCT_CHECK_LAUNCH("gather_tris");
+    return TCT_OK;
+}
+
+extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float *loc32,
+                           const double *obs_org, const double *obs_soa64, int64_t n_src_pad,
+                           const void *rec, const double *src_org, const double *params_host,
+                           double eps2, const tct_tile_cfg *cfg, double *out_partial,
+                           void *stream) {
+    if (int e = check_far_kernel(kernel)) return e;
+    TCT_CHECK_ARG(eps2 >= 0.0, "far_p2p: eps2 must be >= 0");
+    TCT_CHECK_ARG(n_obs_pad > 0 && n_obs_pad % G == 0 && n_src_pad > 0 && n_src_pad % G == 0,
+                  "far_p2p: sizes must be positive multiples of %d", G);
+    TCT_CHECK_ARG(n_obs_pad < (1LL << 31) && n_src_pad < (1LL << 31), "far_p2p: too many points");
+    TCT_CHECK_ARG(rec && out_partial && params_host, "far_p2p: null pointer");
+    TCT_CHECK_ARG(prec == TCT_F64 || (loc32 && obs_org && src_org), "far_p2p: f32 layout null");
+    TCT_CHECK_ARG(prec == TCT_F32 || obs_soa64, "far_p2p: f64 layout null");
+    tct_tile_cfg c{256, 2, 1, 0};
+    if (cfg) c = *cfg;
+    TCT_CHECK_ARG(c.block == 64 || c.block == 128 || c.block == 256, "far_p2p: block %d", c.block);
+    TCT_CHECK_ARG(c.obs_per_lane == 1 || c.obs_per_lane == 2 || c.obs_per_lane == 4,
+                  "far_p2p: obs_per_lane %d", c.obs_per_lane);
+    const int n_tiles = (int)(n_src_pad / G);
+    TCT_CHECK_ARG(c.src_chunks >= 1 && c.src_chunks <= n_tiles && c.src_chunks <= 65535,
+                  "far_p2p: src_chunks %d not in [1, %d]", c.src_chunks, n_tiles);
+    dim3 grid((unsigned)ceil_div(n_obs_pad, (int64_t)c.block * c.obs_per_lane),
+              (unsigned)c.src_chunks);
+    hipStream_t st = as_stream(stream);
+    const int no = (int)n_obs_pad;
+#define TCT_F32_CASE(KID, P)                                                                 \
+    launch_f32<KID, P>(grid, c.block, st, no, loc32, obs_org, n_tiles, (const float *)rec,    \
+                       src_org, kernel_consts<float>(KID, params_host, eps2), out_partial)
+#define TCT_F64_CASE(KID, P)                                                              \
+    launch_f64<KID, P>(grid, c.block, st, no, obs_soa64, n_tiles, (const double *)rec,     \
+                       kernel_consts<double>(KID, params_host, 0.0), out_partial)
+#define TCT_P_SWITCH(MAC, KID)                          \
+    switch (c.obs_per_lane) {                           \
+    case 1: MAC(KID, 1); break;                         \
+    case 2: MAC(KID, 2); break;                         \
+    default: MAC(KID, 4); break;                        \
+    }
+#define TCT_K_SWITCH(MAC)                               \
+    switch (kernel) {                                   \
+    case TCT_U: TCT_P_SWITCH(MAC, TCT_U); break;        \
+    case TCT_T: TCT_P_SWITCH(MAC, TCT_T); break;        \
+    case TCT_A: TCT_P_SWITCH(MAC, TCT_A); break;        \
+    default: TCT_P_SWITCH(MAC, TCT_H); break;           \
+    }
+    if (prec == TCT_F32) {
+        TCT_K_SWITCH(TCT_F32_CASE)
+    } else {
+        TCT_K_SWITCH(TCT_F64_CASE)
+    }
+#undef TCT_K_SWITCH
+#undef TCT_P_SWITCH
+#undef TCT_F64_CASE
+#undef TCT_F32_CASE
+    TCT_CHECK_LAUNCH("far_p2p");
+    return TCT_OK;
+}
+
+extern "C" int tct_far_reduce_pts(int64_t n, int64_t n_obs_pad, int chunks, const double *partial,
+                                  double *out, void *stream) {
+    TCT_CHECK_ARG(n >= 0 && n <= n_obs_pad && chunks >= 1 && partial && out,
+                  "reduce_pts: bad arguments");
+    if (n == 0) return TCT_OK;
+    hipLaunchKernelGGL(reduce_pts_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                       as_stream(stream), n, n_obs_pad, chunks, partial, out);
+    TCT_CHECK_LAUNCH("reduce_pts");
+    return TCT_OK;
+}
+
+extern "C" int tct_far_scatter_tris(int64_t n_tris, int nq, const int32_t *pt_of_tri,
+                                    const double *qphi, const double *wj_obs, int64_t n_obs_pad,
+                                    int chunks, const double *partial, double *out,
+                                    void *stream) {
+    TCT_CHECK_ARG(n_tris >= 0 && nq > 0 && chunks >= 1 && n_tris * nq <= n_obs_pad,
+                  "scatter_tris: bad sizes");
+    TCT_CHECK_ARG(pt_of_tri && qphi && wj_obs && partial && out, "scatter_tris: null pointer");
+    if (n_tris == 0) return TCT_OK;
+    hipLaunchKernelGGL(scatter_tris_kernel, dim3((unsigned)ceil_div(n_tris * 9, 256)), dim3(256),
+                       0, as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, n_obs_pad,
+                       chunks, partial, out);
+    TCT_CHECK_LAUNCH("scatter_tris");
+    return TCT_OK;
+}

@@ -1,0 +1,142 @@
+// Host-side (CPU, OpenMP) near-pair finder.
+//
+// Reference: fast_find_nearfield.get_nearfield (tectosaur/mesh/
+// fast_find_nearfield.cpp:22-141, an octree dual traversal).  The pair set
+// is defined by the leaf test at :76-83:
+//     include (i, j)  iff  dist2(c_i, c_j) <= pow((r_i + r_j) * thr, 2)
+// with dist2 = ((0 + dx*dx) + dy*dy) + dz*dz (fmm/geometry.hpp:7-13,61-64).
+// This implementation finds exactly that set with a uniform grid over the
+// B centroids (cell >= the largest possible reach), evaluates the same
+// predicate with the same floating-point operation order (this file is
+// compiled with -ffp-contract=off), and returns the pairs sorted by (i, j).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/tct_hip.h"
+
+namespace tct {
+int set_error(int code, const char *fmt, ...);
+}
+
+namespace {
+
+inline double dist2(const double *a, const double *b) {
+    double out = 0;
+    for (int d = 0; d < 3; d++) {
+        double t = a[d] - b[d];
+        out += t * t;
+    }
+    return out;
+}
+
+struct Grid {
+    double lo[3];
+    double cell;
+    int64_t dims[3];
+    std::vector<int64_t> start;  // [n_cells + 1]
+    std::vector<int64_t> items;  // B indices grouped by cell
+
+    int64_t coord(double v, int d) const {
+        int64_t c = (int64_t)std::floor((v - lo[d]) / cell);
+        return std::min<int64_t>(std::max<int64_t>(c, 0), dims[d] - 1);
+    }
+};
+
+void build_grid(Grid &g, int64_t n, const double *c, double cell) {
+    double hi[3];
+    for (int d = 0; d < 3; d++) {
+        g.lo[d] = INFINITY;
+        hi[d] = -INFINITY;
+    }
+    for (int64_t i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            g.lo[d] = std::min(g.lo[d], c[3 * i + d]);
+            hi[d] = std::max(hi[d], c[3 * i + d]);
+        }
+    // keep the cell count near n so empty cells cost little
+    double ext = 0;
+    for (int d = 0; d < 3; d++) ext = std::max(ext, hi[d] - g.lo[d]);
+    cell = std::max(cell, ext / 1024.0);
+    if (!(cell > 0)) cell = 1.0;
+    for (;;) {
+        int64_t total = 1;
+        for (int d = 0; d < 3; d++) {
+            g.dims[d] = (int64_t)std::floor((hi[d] - g.lo[d]) / cell) + 1;
+            total *= g.dims[d];
+        }
+        if (total <= std::max<int64_t>(8 * n, 64)) break;
+        cell *= 1.5;
+    }
+    g.cell = cell;
+    const int64_t nc = g.dims[0] * g.dims[1] * g.dims[2];
+    g.start.assign(nc + 1, 0);
+    std::vector<int64_t> cid(n);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t cx = g.coord(c[3 * i], 0), cy = g.coord(c[3 * i + 1], 1),
+                cz = g.coord(c[3 * i + 2], 2);
+        cid[i] = (cz * g.dims[1] + cy) * g.dims[0] + cx;
+        g.start[cid[i] + 1]++;
+    }
+    for (int64_t k = 0; k < nc; k++) g.start[k + 1] += g.start[k];
+    g.items.resize(n);
+    std::vector<int64_t> fill(g.start.begin(), g.start.end() - 1);
+    for (int64_t i = 0; i < n; i++) g.items[fill[cid[i]]++] = i;
+}
+
+}  // namespace
+
+extern "C" int tct_find_near_pairs(int64_t n_a, const double *ca, const double *ra, int64_t n_b,
+                                   const double *cb, const double *rb, double threshold,
+                                   int64_t *out, int64_t cap, int64_t *n_out) {
+    if (n_a < 0 || n_b < 0 || !n_out || (n_a && (!ca || !ra)) || (n_b && (!cb || !rb)) ||
+        !(threshold >= 0))
+        return tct::set_error(TCT_EARG, "find_near_pairs: bad arguments");
+    *n_out = 0;
+    if (n_a == 0 || n_b == 0) return TCT_OK;
+    double rmax_a = 0, rmax_b = 0;
+    for (int64_t i = 0; i < n_a; i++) rmax_a = std::max(rmax_a, ra[i]);
+    for (int64_t j = 0; j < n_b; j++) rmax_b = std::max(rmax_b, rb[j]);
+    Grid g;
+    build_grid(g, n_b, cb, threshold * (rmax_a + rmax_b) * 1.0000001);
+
+    // per-thread pair lists over contiguous i ranges, concatenated in order
+    std::vector<std::vector<int64_t>> per_i(n_a);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n_a; i++) {
+        const double *ci = ca + 3 * i;
+        const double reach = threshold * (ra[i] + rmax_b) * 1.0000001;
+        int64_t lo[3], hi[3];
+        for (int d = 0; d < 3; d++) {
+            lo[d] = g.coord(ci[d] - reach, d);
+            hi[d] = g.coord(ci[d] + reach, d);
+        }
+        std::vector<int64_t> &mine = per_i[i];
+        for (int64_t z = lo[2]; z <= hi[2]; z++)
+            for (int64_t y = lo[1]; y <= hi[1]; y++)
+                for (int64_t x = lo[0]; x <= hi[0]; x++) {
+                    const int64_t cell = (z * g.dims[1] + y) * g.dims[0] + x;
+                    for (int64_t k = g.start[cell]; k < g.start[cell + 1]; k++) {
+                        const int64_t j = g.items[k];
+                        const double lim = std::pow((ra[i] + rb[j]) * threshold, 2);
+                        if (dist2(ci, cb + 3 * j) > lim) continue;
+                        mine.push_back(j);
+                    }
+                }
+        std::sort(mine.begin(), mine.end());
+    }
+    int64_t total = 0;
+    for (int64_t i = 0; i < n_a; i++) total += (int64_t)per_i[i].size();
+    *n_out = total;
+    if (!out || cap < total) return TCT_OK;  // caller retries with cap >= *n_out
+    int64_t k = 0;
+    for (int64_t i = 0; i < n_a; i++)
+        for (int64_t j : per_i[i]) {
+            out[2 * k] = i;
+            out[2 * k + 1] = j;
+            k++;
+        }
+    return TCT_OK;
+}

@@ -1,0 +1,417 @@
+// Nearfield pair integrator, FP64, gfx950.
+//
+// Reference semantics: single_pairs{Z,N} and single_pairsN_adj
+// (tectosaur/assemble.cl:24-103) through integrate_pair
+// (tectosaur/integral_primitives.cl:174-213), driven by PairsIntegrator
+// (tectosaur/nearfield/pairs_integrator.py:27-93).  For each triangle pair
+//   out[b_obs, d_obs, b_src, d_src] =
+//       J_obs J_src sum_q w_q phi_b_obs(x_q) phi_b_src(y_q) K_{d_obs d_src}(y_q - x_q)
+// (regularized kernels: the surface-curl forms of tectosaur/kernels/
+// kernels.cl:29-98), optionally skipping r2 == 0, with the rotation,
+// flip and de-rotation of adjacent pairs.
+//
+// Design:
+//  * G lanes share one pair (G = 1, 4, 16 or 64); each lane takes every
+//    G-th quadrature point, so a large singular rule (thousands of points)
+//    is spread over a whole wavefront ("one workgroup per pair") while a
+//    small correction rule keeps one pair per lane.
+//  * Accumulators hold only what varies with the quadrature point:
+//      U/T/A/H : the 81 basis-weighted kernel entries;
+//      RT/RA   : 30 moments (w phi Q1, w phi Q2 D_p D_t, w phi phi Q3 n.D);
+//      RH      : 7 moments (w/r and w D_i D_j / r^3) -- the surface curls
+//                are constant per triangle and are applied once at the end.
+//  * The G partial sums are combined with an xor-butterfly inside the
+//    group, then one lane forms and writes the 81 outputs (de-rotated).
+#include "tct_common.h"
+
+namespace tct {
+
+struct Tri {
+    double v[3][3];              // vertices in the rotated frame
+    double n[3];                 // unit normal
+    double jac;                  // |unscaled normal| = 2 * area
+    double curl[3][3];           // surface curl of the basis functions
+};
+
+__device__ __forceinline__ int pmod3(int i) { return ((i % 3) + 3) % 3; }
+
+// tri_info of tectosaur/integral_primitives.cl:56-93 (+ surf_curl_basis
+// :95-112): vertex c of the rotated triangle is original vertex
+// (flip(c) + clicks) mod 3; the flip also negates the unscaled normal.
+__device__ __forceinline__ void load_tri(Tri &t, const double *__restrict__ pts,
+                                         const int32_t *__restrict__ tris, int idx, int clicks,
+                                         int flip) {
+    double v[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        int fc = c;
+        if (flip) fc = (c == 0) ? 1 : (c == 1 ? 0 : 2);
+        const int p = tris[3 * idx + pmod3(fc + clicks)];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[c][d] = pts[3 * p + d];
+    }
+    double s20[3], s21[3], un[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        s20[d] = v[2][d] - v[0][d];
+        s21[d] = v[2][d] - v[1][d];
+    }
+    un[0] = s20[1] * s21[2] - s20[2] * s21[1];
+    un[1] = s20[2] * s21[0] - s20[0] * s21[2];
+    un[2] = s20[0] * s21[1] - s20[1] * s21[0];
+    if (flip)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) un[d] = -un[d];
+    t.jac = sqrt(un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
+    const double inv = 1.0 / t.jac;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        t.n[d] = un[d] * inv;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) t.v[c][d] = v[c][d];
+    }
+    // grad of (1-x-y, x, y) = (-1,-1), (1,0), (0,1); curl_b = (g_b0 e2 - g_b1 e1) / J
+    // with e1 = v1 - v0, e2 = v2 - v0.
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const double e1 = v[1][s] - v[0][s], e2 = v[2][s] - v[0][s];
+        t.curl[0][s] = (-e2 + e1) / t.jac;
+        t.curl[1][s] = e2 / t.jac;
+        t.curl[2][s] = -e1 / t.jac;
+    }
+}
+
+// Number of accumulators per kernel class.
+template <int KID>
+constexpr int n_acc() {
+    return KID <= TCT_H ? 81 : (KID == TCT_RH ? 7 : 30);
+}
+
+// 3x3 kernel tensor (tectosaur/kernels/__init__.py tensor_code), m = obs
+// normal, n = src normal, D = y - x.
+template <int KID>
+__device__ __forceinline__ void tensor(const double D[3], double r2, const double m[3],
+                                       const double n[3], const ElasticConsts &c,
+                                       double K[9]) {
+    const double invr = rsqrt(r2);
+    const double invr2 = invr * invr;
+    const double invr3 = invr2 * invr;
+    if constexpr (KID == TCT_U) {
+        const double Q1 = c.CsU0 * invr, Q2 = c.CsU1 * invr3;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) K[i * 3 + j] = Q2 * D[i] * D[j] + (i == j ? Q1 : 0.0);
+    } else if constexpr (KID == TCT_T || KID == TCT_A) {
+        // K = A(-(N.D) I + s W) + C (N.D) D D^T with W the cross-product
+        // matrix of N x D (sign s = +1 for T, -1 for A).
+        const double *N = KID == TCT_T ? n : m;
+        const double s = KID == TCT_T ? 1.0 : -1.0;
+        const double rn = N[0] * D[0] + N[1] * D[1] + N[2] * D[2];
+        const double A = s * c.CsT0 * invr3;
+        const double C = -s * c.CsT1 * invr3 * invr2;
+        const double cx = N[1] * D[2] - N[2] * D[1];
+        const double cy = N[2] * D[0] - N[0] * D[2];
+        const double cz = N[0] * D[1] - N[1] * D[0];
+        const double As = A * s;
+        K[0] = -A * rn + C * rn * D[0] * D[0];
+        K[1] = -As * cz + C * rn * D[0] * D[1];
+        K[2] = As * cy + C * rn * D[0] * D[2];
+        K[3] = As * cz + C * rn * D[1] * D[0];
+        K[4] = -A * rn + C * rn * D[1] * D[1];
+        K[5] = -As * cx + C * rn * D[1] * D[2];
+        K[6] = -As * cy + C * rn * D[2] * D[0];
+        K[7] = As * cx + C * rn * D[2] * D[1];
+        K[8] = -A * rn + C * rn * D[2] * D[2];
+    } else {
+        const double Dr[3] = {D[0] * invr, D[1] * invr, D[2] * invr};
+        const double rn = n[0] * Dr[0] + n[1] * Dr[1] + n[2] * Dr[2];
+        const double rm = m[0] * Dr[0] + m[1] * Dr[1] + m[2] * Dr[2];
+        const double mn = m[0] * n[0] + m[1] * n[1] + m[2] * n[2];
+        const double Q = c.CsH0 * invr3;
+        const double A = Q * 3.0 * rn;
+        const double B = Q * c.CsH1;
+        const double C = Q * c.CsH3;
+        double MT[3], NT[3], DT[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            MT[k] = Q * c.CsH2 * n[k] + A * c.CsH1 * Dr[k];
+            NT[k] = B * m[k] + C * Dr[k] * rm;
+            DT[k] = B * 3.0 * n[k] * rm + C * Dr[k] * mn + A * (c.nu * m[k] - 5.0 * Dr[k] * rm);
+        }
+        const double ST = A * c.nu * rm + B * mn;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                K[i * 3 + j] = n[i] * NT[j] + m[i] * MT[j] + Dr[i] * DT[j] + (i == j ? ST : 0.0);
+    }
+}
+
+// Symmetric index of (p, t) in 6-vector order xx, xy, xz, yy, yz, zz.
+__device__ __forceinline__ constexpr int sym6(int p, int t) {
+    return p <= t ? (p == 0 ? t : (p == 1 ? 2 + t : 5)) : (t == 0 ? p : (t == 1 ? 2 + p : 5));
+}
+
+template <int KID>
+__device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, const double *q,
+                                                 double w, int check0, const ElasticConsts &c,
+                                                 double *acc) {
+    const double ox = q[0], oy = q[1], sx = q[2], sy = q[3];
+    const double po[3] = {1.0 - ox - oy, ox, oy};
+    const double ps[3] = {1.0 - sx - sy, sx, sy};
+    // basis-weighted vertex sums, as pts_from_basis (integral_primitives.cl:121-128)
+    double D[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double x = po[0] * ot.v[0][d] + po[1] * ot.v[1][d] + po[2] * ot.v[2][d];
+        const double y = ps[0] * st.v[0][d] + ps[1] * st.v[1][d] + ps[2] * st.v[2][d];
+        D[d] = y - x;
+    }
+    const double r2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+    if (check0 && r2 == 0.0) return;
+    if constexpr (KID <= TCT_H) {
+        double K[9];
+        tensor<KID>(D, r2, ot.n, st.n, c, K);
+#pragma unroll
+        for (int bo = 0; bo < 3; ++bo)
+#pragma unroll
+            for (int bs = 0; bs < 3; ++bs) {
+                const double f = w * po[bo] * ps[bs];
+#pragma unroll
+                for (int dob = 0; dob < 3; ++dob)
+#pragma unroll
+                    for (int ds = 0; ds < 3; ++ds)
+                        acc[bo * 27 + dob * 9 + bs * 3 + ds] += f * K[dob * 3 + ds];
+            }
+    } else if constexpr (KID == TCT_RH) {
+        const double invr = rsqrt(r2);
+        const double wi = w * invr;
+        const double wi3 = wi * invr * invr;
+        acc[0] += wi;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int t = p; t < 3; ++t) acc[1 + sym6(p, t)] += wi3 * D[p] * D[t];
+    } else {
+        // RT (weights follow the obs basis, normal = src) or RA (src basis, obs normal).
+        const double invr = rsqrt(r2);
+        const double invr3 = invr * invr * invr;
+        const double *nn = KID == TCT_RT ? st.n : ot.n;
+        const double *pL = KID == TCT_RT ? po : ps;
+        const double Q1 = c.CsRT1 * invr;
+        const double Q2 = c.CsRT0 * invr3;
+        const double Q3nD = c.CsRT2 * invr3 * (nn[0] * D[0] + nn[1] * D[1] + nn[2] * D[2]);
+        double DD[6];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int t = p; t < 3; ++t) DD[sym6(p, t)] = D[p] * D[t];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const double wl = w * pL[b];
+            acc[b] += wl * Q1;
+            const double wq = wl * Q2;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[3 + 6 * b + k] += wq * DD[k];
+        }
+#pragma unroll
+        for (int bo = 0; bo < 3; ++bo)
+#pragma unroll
+            for (int bs = 0; bs < 3; ++bs) acc[21 + 3 * bo + bs] += w * po[bo] * ps[bs] * Q3nD;
+    }
+}
+
+__device__ __forceinline__ int levi(int i, int j, int k) { return (i - j) * (j - k) * (k - i) / 2; }
+
+// Output value (rotated-frame indices bo, dob, bs, ds) from the reduced
+// accumulators, before the J_obs J_src factor.
+template <int KID>
+__device__ __forceinline__ double final_value(const double *acc, const Tri &ot, const Tri &st,
+                                              const ElasticConsts &c, int bo, int dob, int bs,
+                                              int ds) {
+    if constexpr (KID <= TCT_H) {
+        return acc[bo * 27 + dob * 9 + bs * 3 + ds];
+    } else if constexpr (KID == TCT_RH) {
+        const double *co = ot.curl[bo];
+        const double *cs = st.curl[bs];
+        const double cc = co[0] * cs[0] + co[1] * cs[1] + co[2] * cs[2];
+        const double S0 = acc[0];
+        const double S2 = acc[1 + sym6(dob, ds)];
+        return c.CsRH0 * (-2.0 * S2 * cc +
+                          S0 * (c.CsRH1 * co[dob] * cs[ds] + c.CsRH2 * co[ds] * cs[dob] -
+                                (dob == ds ? 2.0 * cc : 0.0)));
+    } else if constexpr (KID == TCT_RT) {
+        // sum_Ij curl_s[bs][Ij] (e[Ij][do][ds] A1[bo] + sum_t e[Ij][t][ds] A2[bo][do,t])
+        //   + delta(do, ds) A3[bo][bs]
+        double v = (dob == ds) ? acc[21 + 3 * bo + bs] : 0.0;
+        for (int Ij = 0; Ij < 3; ++Ij) {
+            double g = levi(Ij, dob, ds) * acc[bo];
+            for (int t = 0; t < 3; ++t) g += levi(Ij, t, ds) * acc[3 + 6 * bo + sym6(dob, t)];
+            v += st.curl[bs][Ij] * g;
+        }
+        return v;
+    } else {
+        // RA: sum_Ij curl_o[bo][Ij] (e[Ij][ds][do] B1[bs] + sum_t e[Ij][t][do] B2[bs][ds,t])
+        //   - delta(do, ds) B3[bo][bs]
+        double v = (dob == ds) ? -acc[21 + 3 * bo + bs] : 0.0;
+        for (int Ij = 0; Ij < 3; ++Ij) {
+            double g = levi(Ij, ds, dob) * acc[bs];
+            for (int t = 0; t < 3; ++t) g += levi(Ij, t, dob) * acc[3 + 6 * bs + sym6(ds, t)];
+            v += ot.curl[bo][Ij] * g;
+        }
+        return v;
+    }
+}
+
+template <int KID, int GL>
+__global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__restrict__ qx,
+                                                    const double *__restrict__ qw,
+                                                    const double *__restrict__ pts,
+                                                    const int32_t *__restrict__ tris, int n_pairs,
+                                                    const int32_t *__restrict__ pairs,
+                                                    int pair_width, int check0, ElasticConsts c,
+                                                    double *__restrict__ out) {
+    constexpr int NA = n_acc<KID>();
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pair = gid / GL;
+    const int sub = gid % GL;
+    // Lanes of a group past the end still take part in the butterfly.
+    const bool live = pair < n_pairs;
+    const int pr = live ? pair : n_pairs - 1;
+    const int32_t *pp = pairs + (size_t)pr * pair_width;
+    int oc = 0, sc = 0, flip = 0;
+    if (pair_width == 5) {
+        oc = pp[2];
+        sc = pp[3];
+        flip = pp[4] != 0;
+    }
+    Tri ot, st;
+    load_tri(ot, pts, tris, pp[0], oc, 0);
+    load_tri(st, pts, tris, pp[1], sc, flip);
+
+    double acc[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    if (live)
+        for (int iq = sub; iq < nq; iq += GL)
+            accumulate_point<KID>(ot, st, qx + 4 * (size_t)iq, qw[iq], check0, c, acc);
+    if constexpr (GL > 1) {
+#pragma unroll
+        for (int k = 0; k < NA; ++k) {
+            double v = acc[k];
+#pragma unroll
+            for (int off = GL / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            acc[k] = v;
+        }
+    }
+    if (!live || sub != 0) return;
+
+    // De-rotation (assemble.cl:77-101): out[b1] = res[od[b1]] with
+    // od[i] = (i - clicks) mod 3; for the source, entries 0 and 1 of the
+    // map are swapped when the source was flipped.  Written here as the
+    // inverse map applied to the output address, so every register index
+    // stays static.
+    int sd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sd[i] = pmod3(i - sc);
+    if (flip) {
+        const int tmp = sd[0];
+        sd[0] = sd[1];
+        sd[1] = tmp;
+    }
+    int oinv[3], sinv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        oinv[i] = pmod3(i + oc);
+        sinv[i] = (sd[0] == i) ? 0 : ((sd[1] == i) ? 1 : 2);
+    }
+    const double J = ot.jac * st.jac;
+    double *o = out + (size_t)pair * 81;
+#pragma unroll
+    for (int i1 = 0; i1 < 3; ++i1)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+            for (int i2 = 0; i2 < 3; ++i2)
+#pragma unroll
+                for (int d2 = 0; d2 < 3; ++d2)
+                    o[oinv[i1] * 27 + d1 * 9 + sinv[i2] * 3 + d2] =
+                        J * final_value<KID>(acc, ot, st, c, i1, d1, i2, d2);
+}
+
+__global__ void sub_f64_kernel(int64_t n, const double *__restrict__ a,
+                               const double *__restrict__ b, double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] - b[i];
+}
+
+template <int KID, int GL>
+static void launch_pairs(hipStream_t st, int nq, const double *qx, const double *qw,
+                         const double *pts, const int32_t *tris, int n_pairs,
+                         const int32_t *pairs, int pair_width, int check0,
+                         const ElasticConsts &c, double *out) {
+    const int64_t threads = (int64_t)n_pairs * GL;
+    const unsigned nb = (unsigned)ceil_div(threads, 256);
+    hipLaunchKernelGGL((pairs_kernel<KID, GL>), dim3(nb), dim3(256), 0, st, nq, qx, qw, pts, tris,
+                       n_pairs, pairs, pair_width, check0, c, out);
+}
+
+template <int KID>
+static void launch_pairs_g(int gl, hipStream_t st, int nq, const double *qx, const double *qw,
+                           const double *pts, const int32_t *tris, int n_pairs,
+                           const int32_t *pairs, int pair_width, int check0,
+                           const ElasticConsts &c, double *out) {
+    switch (gl) {
+    case 1: launch_pairs<KID, 1>(st, nq, qx, qw, pts, tris, n_pairs, pairs, pair_width, check0, c, out); break;
+    case 4: launch_pairs<KID, 4>(st, nq, qx, qw, pts, tris, n_pairs, pairs, pair_width, check0, c, out); break;
+    case 16: launch_pairs<KID, 16>(st, nq, qx, qw, pts, tris, n_pairs, pairs, pair_width, check0, c, out); break;
+    default: launch_pairs<KID, 64>(st, nq, qx, qw, pts, tris, n_pairs, pairs, pair_width, check0, c, out); break;
+    }
+}
+
+}  // namespace tct
+
+using namespace tct;
+
+extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, const double *qw,
+                                   const double *pts, const int32_t *tris, int64_t n_pairs,
+                                   const int32_t *pairs, int pair_width, int check0,
+                                   const double *params_host, int lanes_per_pair, double *out,
+                                   void *stream) {
+    TCT_CHECK_ARG(kernel >= TCT_U && kernel <= TCT_RH, "pairs: kernel id %d", kernel);
+    TCT_CHECK_ARG(nq > 0 && nq < (1LL << 31), "pairs: nq %lld", (long long)nq);
+    TCT_CHECK_ARG(n_pairs >= 0 && n_pairs < (1LL << 26), "pairs: n_pairs %lld (max 2^26 per call)",
+                  (long long)n_pairs);
+    TCT_CHECK_ARG(pair_width == 2 || pair_width == 5, "pairs: pair_width %d", pair_width);
+    TCT_CHECK_ARG(qx && qw && pts && tris && params_host && out && (pairs || n_pairs == 0),
+                  "pairs: null pointer");
+    if (n_pairs == 0) return TCT_OK;
+    int gl = lanes_per_pair;
+    if (gl == 0) gl = nq >= 1024 ? 64 : (nq >= 128 ? 16 : (nq >= 24 ? 4 : 1));
+    TCT_CHECK_ARG(gl == 1 || gl == 4 || gl == 16 || gl == 64, "pairs: lanes_per_pair %d", gl);
+    const ElasticConsts c = elastic_consts(params_host);
+    hipStream_t st = as_stream(stream);
+    const int np = (int)n_pairs, q = (int)nq;
+    switch (kernel) {
+    case TCT_U: launch_pairs_g<TCT_U>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
+    case TCT_T: launch_pairs_g<TCT_T>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
+    case TCT_A: launch_pairs_g<TCT_A>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
+    case TCT_H: launch_pairs_g<TCT_H>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
+    case TCT_RT: launch_pairs_g<TCT_RT>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
+    case TCT_RA: launch_pairs_g<TCT_RA>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
+    default: launch_pairs_g<TCT_RH>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
+    }
+    TCT_CHECK_LAUNCH("pairs_integrate");
+    return TCT_OK;
+}
+
+extern "C" int tct_sub_f64(int64_t n, const double *a, const double *b, double *out,
+                           void *stream) {
+    TCT_CHECK_ARG(n >= 0 && a && b && out, "sub_f64: bad arguments");
+    if (n == 0) return TCT_OK;
+    hipLaunchKernelGGL(sub_f64_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                       as_stream(stream), n, a, b, out);
+    TCT_CHECK_LAUNCH("sub_f64");
+    return TCT_OK;
+}

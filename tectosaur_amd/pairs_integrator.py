@@ -1,0 +1,100 @@
+"""``PairsIntegrator``: the nearfield pair operator, drop-in for
+tectosaur/nearfield/pairs_integrator.py:27-93.
+
+Every method integrates a 4-D quadrature rule over a list of triangle pairs
+on the device (pairs.hip, FP64) and returns [n, 3, 3, 3, 3] blocks laid out
+[b_obs, d_obs, b_src, d_src].  The nearfield is computed in FP64 whatever
+``float_type`` says (the reference computes it in float_type with Kahan
+summation); results are returned as float64.
+
+The ``*_dev`` variants take and return device tensors, for assembling the
+nearfield matrix without host round trips.
+"""
+import numpy as np
+import torch
+
+from . import _native as N
+from . import triangle_rules
+from .quadrature import gauss4d_tri
+
+# pairs per launch: bounds the output buffer of one call (81 doubles/pair)
+CALL_SIZE = 1 << 20
+
+
+class PairsIntegrator:
+    def __init__(self, kernel, params, float_type, nq_far, nq_near, pts, tris):
+        self.kernel = kernel
+        self.kid = N.KERNEL_IDS[kernel]
+        self.float_type = float_type
+        self.params = N.params_array(params)
+        self.device = N.require_device()
+        f64 = dict(dtype=torch.float64, device=self.device)
+        self.gpu_pts = torch.as_tensor(np.ascontiguousarray(pts, dtype=np.float64), **f64)
+        self.gpu_tris = torch.as_tensor(np.ascontiguousarray(tris, dtype=np.int32),
+                                        device=self.device)
+        self.near_q = self.quad_to_gpu(gauss4d_tri(nq_near, nq_near))
+        self.far_q = self.quad_to_gpu(gauss4d_tri(nq_far, nq_far))
+        self._rules = {}
+
+    def quad_to_gpu(self, q):
+        f64 = dict(dtype=torch.float64, device=self.device)
+        return (torch.as_tensor(np.ascontiguousarray(q[0]), **f64),
+                torch.as_tensor(np.ascontiguousarray(q[1]), **f64))
+
+    def _rule(self, key, make):
+        if key not in self._rules:
+            self._rules[key] = self.quad_to_gpu(make())
+        return self._rules[key]
+
+    def pairs_quad_dev(self, q, pairs_dev, check0, out=None, lanes_per_pair=0):
+        """pairs_dev: int32 device tensor [n, 2] or [n, 5]; returns [n, 81]."""
+        n = pairs_dev.shape[0]
+        width = pairs_dev.shape[1] if pairs_dev.dim() == 2 else 2
+        if out is None:
+            out = torch.empty((n, 81), dtype=torch.float64, device=self.device)
+        st = N.stream_ptr()
+        for s in range(0, n, CALL_SIZE):
+            e = min(n, s + CALL_SIZE)
+            N.call('tct_pairs_integrate', self.kid, q[0].shape[0], N.ptr(q[0]), N.ptr(q[1]),
+                   N.ptr(self.gpu_pts), N.ptr(self.gpu_tris), e - s, N.ptr(pairs_dev[s:e]),
+                   width, int(bool(check0)), self.params.ctypes.data, lanes_per_pair,
+                   N.ptr(out[s:e]), st)
+        return out
+
+    def pairs_quad(self, q, pairs_list, check0):
+        pairs_list = np.asarray(pairs_list)
+        n = pairs_list.shape[0]
+        if n == 0:
+            return np.empty((0, 3, 3, 3, 3), dtype=np.float64)
+        p = torch.as_tensor(np.ascontiguousarray(pairs_list.astype(np.int32)),
+                            device=self.device)
+        return self.pairs_quad_dev(q, p, check0).cpu().numpy().reshape((n, 3, 3, 3, 3))
+
+    def correction(self, pairs_list, check0):
+        return self.pairs_quad(self.far_q, np.asarray(pairs_list)[:, :2], check0)
+
+    def nearfield(self, pairs_list):
+        return self.pairs_quad(self.near_q, pairs_list, False)
+
+    def vert_adj(self, nq, pairs_list):
+        if isinstance(nq, (int, np.integer)):
+            nq = (nq, nq, nq)
+        q = self._rule(('va',) + tuple(nq), lambda: triangle_rules.vertex_adj_quad(*nq))
+        return self.pairs_quad(q, self._width5(pairs_list), True)
+
+    def coincident(self, nq, pairs_list):
+        q = self._rule(('co', nq), lambda: triangle_rules.coincident_quad(nq))
+        return self.pairs_quad(q, pairs_list, True)
+
+    def edge_adj(self, nq, pairs_list):
+        q = self._rule(('ea', nq), lambda: triangle_rules.edge_adj_quad(nq))
+        return self.pairs_quad(q, self._width5(pairs_list), True)
+
+    @staticmethod
+    def _width5(pairs_list):
+        p = np.asarray(pairs_list).astype(np.int64)
+        if p.shape[0] == 0:
+            return np.zeros((0, 5), dtype=np.int64)
+        if p.shape[1] == 4:  # (obs, src, obs_clicks, src_clicks) with no flip column
+            p = np.concatenate([p, np.zeros((p.shape[0], 1), dtype=np.int64)], axis=1)
+        return p

@@ -1,0 +1,114 @@
+"""``TriToTriDirectFarfieldOp``: the farfield operator of the summed matvec,
+drop-in for tectosaur/ops/sparse_farfield_op.py:16-71.
+
+The reference launches one thread per observation triangle that loops over
+every source triangle and every quadrature-point pair
+(farfield_tris_to_tris{K}, tectosaur/matrix_free.cl:19-107).  Here the same
+sum is computed as three device steps per matvec:
+
+  1. gather   s_j = J_t w_q sum_b phi_b(q) v[9t+3b+:]  (one value per source
+               quadrature point; the factor J_t w_q of matrix_free.cl:83 and
+               the interpolation of :84-90 folded together),
+  2. N-body   r_i = sum_j K(y_j - x_i) s_j             (far_p2p.hip),
+  3. scatter  out[9t+3b+:] = J_t sum_q w_q phi_b(q) r_(t,q).
+
+For the point kernels U, T, A, H this is the same sum in a different order
+(the kernel of a pair depends only on the two points and normals).  The
+surface-curl kernels RT, RA, RH have no point form and are not supported
+by this operator.
+"""
+import numpy as np
+import torch
+
+from . import _native as N
+from .farfield import FarfieldEngine, morton_order, prec_of
+from .kernels import kernels as KERNELS
+from .quadrature import gauss2d_tri
+
+
+class TriToTriDirectFarfieldOp:
+    def __init__(self, nq_far, K_name, params, pts, tris, float_type, obs_subset, src_subset,
+                 tile_cfg=None, precisions=None):
+        K = KERNELS[K_name]
+        if K.surf_curl_obs or K.surf_curl_src:
+            raise NotImplementedError('%s: surface-curl farfield is not supported' % K_name)
+        pts = np.asarray(pts, dtype=np.float64)
+        tris = np.asarray(tris)
+        obs_subset = np.asarray(obs_subset, dtype=np.int64)
+        src_subset = np.asarray(src_subset, dtype=np.int64)
+        self.shape = (obs_subset.shape[0] * 9, src_subset.shape[0] * 9)
+        self.dim = 3
+        self.tensor_dim = K.tensor_dim
+        self.n_obs = obs_subset.shape[0]
+        self.n_src = src_subset.shape[0]
+        self.float_type = float_type
+        self.prec = prec_of(float_type)
+        self.K_name = K_name
+        self.q = gauss2d_tri(nq_far)
+        self.nq = self.q[0].shape[0]
+        qx, qw = self.q
+        phi = np.stack([1 - qx[:, 0] - qx[:, 1], qx[:, 0], qx[:, 1]], axis=1)
+        dev = N.require_device()
+        self.device = dev
+
+        def side(subset):
+            t = tris[subset]
+            v = pts[t]
+            order = morton_order(v.mean(axis=1))
+            v = v[order]
+            un = np.cross(v[:, 2] - v[:, 0], v[:, 2] - v[:, 1])
+            jac = np.linalg.norm(un, axis=1)
+            n = un / jac[:, None]
+            qp = np.einsum('qb,tbd->tqd', phi, v).reshape((-1, 3))
+            qn = np.repeat(n, self.nq, axis=0)
+            wj = (jac[:, None] * qw[None, :]).reshape(-1)
+            return order, qp, qn, wj
+
+        o_order, o_pts, o_ns, o_wj = side(obs_subset)
+        s_order, s_pts, s_ns, s_wj = side(src_subset)
+        if precisions is None:
+            precisions = (self.prec,)
+        self.engine = FarfieldEngine(K_name, params, o_pts, o_ns, s_pts, s_ns,
+                                     precisions=precisions, device=dev, tile_cfg=tile_cfg)
+        i32 = dict(dtype=torch.int32, device=dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        # source point k belongs to DOF-space triangle s_order[k // nq]
+        self.tri_of_pt = torch.as_tensor(np.repeat(s_order, self.nq).astype(np.int32), **i32)
+        pt_of_tri = np.empty(self.n_obs, dtype=np.int32)
+        pt_of_tri[o_order] = np.arange(self.n_obs, dtype=np.int32)
+        self.pt_of_tri = torch.as_tensor(pt_of_tri, **i32)
+        self.qphi = torch.as_tensor(np.ascontiguousarray(phi), **f64)
+        self.wj_src = torch.as_tensor(s_wj, **f64)
+        self.wj_obs = torch.as_tensor(o_wj, **f64)
+
+    @property
+    def pair_evals(self):
+        return self.engine.pair_evals
+
+    def dot_dev(self, v_dev, out_dev=None, prec=None):
+        """Device matvec: v_dev float64 [n_src*9] -> float64 [n_obs*9]."""
+        prec = prec or self.prec
+        if out_dev is None:
+            out_dev = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+        e = self.engine
+        e.gather_tris(v_dev, self.tri_of_pt, self.qphi, self.wj_src, self.nq, prec)
+        e.run(prec)
+        e.scatter_tris(prec, self.n_obs, self.nq, self.pt_of_tri, self.qphi, self.wj_obs,
+                       out_dev)
+        return out_dev
+
+    def dot(self, v):
+        v_dev = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device=self.device)
+        return self.dot_dev(v_dev).cpu().numpy()
+
+    async def async_dot(self, v):
+        return self.dot(v)
+
+    def nearfield_dot(self, v):
+        return self.dot(v)
+
+    def nearfield_no_correction_dot(self, v):
+        return self.dot(v)
+
+    def farfield_dot(self, v):
+        return self.dot(v)

@@ -1,0 +1,181 @@
+"""Parity of the HIP farfield path (libtct_hip.so) with the CPU oracle.
+
+Tolerances: FP64 path relative L2 <= 1e-12 (same formula, different
+summation order); FP32 path relative L2 <= 1e-5 (the bar the farfield
+workloads state for FP32 evaluation with FP64 accumulation)."""
+import numpy as np
+import pytest
+
+from tectosaur_amd import mesh
+from tectosaur_amd.quadrature import gauss2d_tri
+
+pytestmark = pytest.mark.gpu
+KS = ['elasticU3', 'elasticT3', 'elasticA3', 'elasticH3']
+TOL = {'f32': 1e-5, 'f64': 1e-12}
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def random_pts(n, seed):
+    rng = np.random.default_rng(seed)
+    obs = rng.random((n, 3))
+    ns = rng.random((n, 3))
+    ns /= np.linalg.norm(ns, axis=1)[:, None]
+    src = rng.random((n, 3))
+    w = rng.random((n, 3)).reshape(-1)
+    return obs, ns, src, ns.copy(), w
+
+
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+@pytest.mark.parametrize('K', KS)
+def test_farfield_pts_direct(oracle, K, prec):
+    # the reference's N-body smoke case (tests/test_farfield.py:53-92): n = 1000 random points
+    from tectosaur_amd.farfield import farfield_pts_direct
+    obs, on, src, sn, w = random_pts(1000, 100)
+    ft = np.float32 if prec == 'f32' else np.float64
+    out = farfield_pts_direct(K, obs, on, src, sn, w, [1.0, 0.25], ft)
+    ref = oracle.farfield_pts(K, obs, on, src, sn, w, [1.0, 0.25])
+    assert rel_l2(out, ref) <= TOL[prec]
+
+
+@pytest.mark.parametrize('n_obs,n_src', [(1, 1), (1, 200), (63, 65), (130, 7)])
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+def test_ragged_sizes(oracle, n_obs, n_src, prec):
+    from tectosaur_amd.farfield import farfield_pts_direct
+    rng = np.random.default_rng(n_obs * 1000 + n_src)
+    obs = rng.random((n_obs, 3)); src = rng.random((n_src, 3))
+    on = rng.normal(size=(n_obs, 3)); on /= np.linalg.norm(on, axis=1)[:, None]
+    sn = rng.normal(size=(n_src, 3)); sn /= np.linalg.norm(sn, axis=1)[:, None]
+    w = rng.normal(size=n_src * 3)
+    ft = np.float32 if prec == 'f32' else np.float64
+    out = farfield_pts_direct('elasticH3', obs, on, src, sn, w, [3e10, 0.25], ft)
+    ref = oracle.farfield_pts('elasticH3', obs, on, src, sn, w, [3e10, 0.25])
+    assert rel_l2(out, ref) <= TOL[prec]
+
+
+def test_coincident_points_skipped(oracle):
+    # r2 == 0 pairs are skipped (farfield_direct.cl:89-91): obs set == src set
+    from tectosaur_amd.farfield import farfield_pts_direct
+    obs, on, _, _, w = random_pts(300, 7)
+    for prec, ft in (('f32', np.float32), ('f64', np.float64)):
+        out = farfield_pts_direct('elasticU3', obs, on, obs, on, w, [1.0, 0.25], ft)
+        ref = oracle.farfield_pts('elasticU3', obs, on, obs, on, w, [1.0, 0.25])
+        assert np.all(np.isfinite(out))
+        assert rel_l2(out, ref) <= TOL[prec]
+
+
+def two_surfaces(n=8, sep=2.0, w=1.0):
+    # tectosaur tests/test_farfield.py:10-25
+    m1 = mesh.make_rect(n, n, [[-w, 0, w], [-w, 0, -w], [w, 0, -w], [w, 0, w]])
+    m2 = mesh.make_rect(n, n, [[-w, sep, w], [-w, sep, -w], [w, sep, -w], [w, sep, w]])
+    m = mesh.concat(m1, m2)
+    s1 = np.arange(m1[1].shape[0])
+    s2 = s1[-1] + 1 + s1
+    return m, s1, s2
+
+
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+@pytest.mark.parametrize('K', KS)
+@pytest.mark.parametrize('nq', [1, 2])
+def test_tri_tri_separated(oracle, K, prec, nq):
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    (pts, tris), s1, s2 = two_surfaces()
+    ft = np.float32 if prec == 'f32' else np.float64
+    op = TriToTriDirectFarfieldOp(nq, K, [1.0, 0.25], pts, tris, ft, s1, s2)
+    v = np.random.default_rng(1).random(op.shape[1])
+    out = op.dot(v)
+    qx, qw = gauss2d_tri(nq)
+    ref = oracle.farfield_tris(K, qx, qw, pts, tris[s1], tris[s2], v, [1.0, 0.25])
+    assert rel_l2(out, ref) <= TOL[prec]
+
+
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+@pytest.mark.parametrize('K', KS)
+def test_tri_tri_self(oracle, K, prec):
+    # obs == src: the coincident quadrature-point pairs are skipped
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = mesh.make_rect(12, 9, [[-1, 0, 1], [-1, 0, -1], [1, 0, -1], [1, 0, 1]])
+    n = tris.shape[0]
+    ft = np.float32 if prec == 'f32' else np.float64
+    op = TriToTriDirectFarfieldOp(2, K, [3e10, 0.25], pts, tris, ft, np.arange(n), np.arange(n))
+    v = np.random.default_rng(2).normal(size=op.shape[1])
+    out = op.dot(v)
+    qx, qw = gauss2d_tri(2)
+    ref = oracle.farfield_tris(K, qx, qw, pts, tris, tris, v, [3e10, 0.25])
+    assert rel_l2(out, ref) <= TOL[prec]
+
+
+def test_tri_tri_overlapping_subsets(oracle):
+    # obs and src subsets overlap but differ: shared triangles are packed into
+    # different groups, and their self pairs must still be skipped in FP32.
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = mesh.make_rect(10, 10, [[-1, 0, 1], [-1, 0, -1], [1, 0, -1], [1, 0, 1]])
+    n = tris.shape[0]
+    obs = np.arange(n // 2 + 20)
+    src = np.arange(n // 2 - 20, n)
+    for prec, ft in (('f32', np.float32), ('f64', np.float64)):
+        op = TriToTriDirectFarfieldOp(2, 'elasticH3', [1.0, 0.25], pts, tris, ft, obs, src)
+        v = np.random.default_rng(3).normal(size=op.shape[1])
+        out = op.dot(v)
+        qx, qw = gauss2d_tri(2)
+        ref = oracle.farfield_tris('elasticH3', qx, qw, pts, tris[obs], tris[src], v, [1.0, 0.25])
+        assert rel_l2(out, ref) <= TOL[prec]
+
+
+def cfg1_mesh():
+    # vertical fault 2 km x 2 km, 32 x 32 squares -> 2048 triangles
+    return mesh.make_rect(33, 33, [[-1000, 0, 0], [1000, 0, 0], [1000, 0, -2000],
+                                   [-1000, 0, -2000]])
+
+
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+def test_cfg1_u_farfield(oracle, prec):
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = cfg1_mesh()
+    n = tris.shape[0]
+    ft = np.float32 if prec == 'f32' else np.float64
+    op = TriToTriDirectFarfieldOp(1, 'elasticU3', [3e10, 0.25], pts, tris, ft, np.arange(n),
+                                  np.arange(n))
+    v = np.random.default_rng(0).standard_normal(18432)
+    out = op.dot(v)
+    qx, qw = gauss2d_tri(1)
+    ref = oracle.farfield_tris('elasticU3', qx, qw, pts, tris, tris, v, [3e10, 0.25])
+    assert rel_l2(out, ref) <= TOL[prec]
+
+
+@pytest.mark.parametrize('cfg', [(64, 1, 1), (128, 2, 3), (256, 4, 7), (256, 1, 32)])
+def test_tile_shapes_agree(cfg):
+    # every launch shape computes the same sum; FP64 results agree to roundoff
+    from tectosaur_amd import _native as N
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = cfg1_mesh()
+    n = tris.shape[0]
+    v = np.random.default_rng(9).normal(size=n * 9)
+    base = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, np.float64,
+                                    np.arange(n), np.arange(n)).dot(v)
+    for ft, tol in ((np.float64, 1e-13), (np.float32, 1e-5)):
+        op = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, ft, np.arange(n),
+                                      np.arange(n), tile_cfg=N.TileCfg(*cfg, 0))
+        assert rel_l2(op.dot(v), base) <= tol
+
+
+def test_deterministic():
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = cfg1_mesh()
+    n = tris.shape[0]
+    op = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, np.float32,
+                                  np.arange(n), np.arange(n))
+    v = np.random.default_rng(10).normal(size=n * 9)
+    a = op.dot(v)
+    b = op.dot(v)
+    assert np.array_equal(a, b)
+
+
+def test_unsupported_kernel_raises():
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = cfg1_mesh()
+    with pytest.raises(NotImplementedError):
+        TriToTriDirectFarfieldOp(1, 'elasticRH3', [3e10, 0.25], pts, tris, np.float32,
+                                 np.arange(4), np.arange(4))

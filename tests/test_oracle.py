@@ -1,0 +1,189 @@
+"""Pins of the CPU oracle: reference golden masters, the sympy derivation of
+the Kelvin kernels, and properties the mathematics fixes.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from tectosaur_amd.quadrature import gauss2d_tri, gauss4d_tri
+from tectosaur_amd.triangle_rules import vertex_adj_quad, coincident_quad, edge_adj_quad
+from tectosaur_amd import mesh
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+KS = ['elasticU3', 'elasticT3', 'elasticA3', 'elasticH3']
+
+
+@pytest.mark.parametrize('K', KS)
+def test_kernel_tensor_matches_sympy(oracle, K):
+    g = np.load(os.path.join(GOLD, 'kernels_golden.npz'))
+    for t in range(g['mu'].shape[0]):
+        D = g['src'][t] - g['obs'][t]
+        Kt = oracle.kernel_tensor(K, D, g['nobs'][t], g['nsrc'][t], [g['mu'][t], g['nu'][t]])
+        np.testing.assert_allclose(Kt, g[K][t], rtol=1e-12, atol=1e-12 * np.abs(g[K][t]).max())
+
+
+@pytest.mark.parametrize('K', KS)
+def test_kernel_vector_equals_tensor(oracle, K):
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        D = rng.normal(size=3)
+        m = rng.normal(size=3); m /= np.linalg.norm(m)
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        v = rng.normal(size=3)
+        Kt = oracle.kernel_tensor(K, D, m, n, [3e10, 0.25])
+        kv = oracle.kernel_vector(K, D, m, n, v, [3e10, 0.25])
+        np.testing.assert_allclose(kv, Kt @ v, rtol=1e-12, atol=1e-14 * np.abs(Kt @ v).max())
+
+
+@pytest.mark.parametrize('K', KS)
+def test_kernel_scaling_law(oracle, K):
+    # K(lambda D) = lambda^(scale_type) K(D): U ~ 1/r, T, A ~ 1/r^2, H ~ 1/r^3
+    st = {'elasticU3': -1, 'elasticT3': -2, 'elasticA3': -2, 'elasticH3': -3}[K]
+    rng = np.random.default_rng(4)
+    D = rng.normal(size=3)
+    m = np.array([0, 0, 1.0]); n = np.array([0.6, 0, 0.8])
+    K1 = oracle.kernel_tensor(K, D, m, n, [1.0, 0.3])
+    K2 = oracle.kernel_tensor(K, 2.5 * D, m, n, [1.0, 0.3])
+    np.testing.assert_allclose(K2, 2.5 ** st * K1, rtol=1e-13, atol=1e-15)
+
+
+def test_farfield_two_tris_golden(oracle):
+    # tectosaur tests/test_integral_op.py:82-94 (H, nq = 3, computed in FP32)
+    pts = np.array([[1, 0, 0], [2, 0, 0], [1, 1, 0], [5, 0, 0], [6, 0, 0], [5, 1, 0]], float)
+    tris = np.array([[0, 1, 2], [3, 4, 5]])
+    qx, qw = gauss4d_tri(3, 3)
+    out = oracle.pairs_integrate('elasticH3', qx, qw, pts, tris, np.array([[0, 1]]), False,
+                                 [1.0, 0.25])
+    g = np.load(os.path.join(GOLD, 'test_farfield_two_tris.npy'))
+    np.testing.assert_almost_equal(out.reshape(g.shape), g, 6)  # the reference's own bar
+    np.testing.assert_allclose(out.reshape(g.shape), g, rtol=0, atol=2e-6 * np.abs(g).max())
+
+
+def test_vert_adjacent_golden(oracle):
+    # tectosaur tests/test_integral_op.py:96-103; the reference passes 4 columns
+    # to a kernel reading 5, the fixture corresponds to src_flip = 0.
+    pts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, -1, 0], [2, 0, 0]], float)
+    tris = np.array([[1, 2, 0], [1, 3, 4]])
+    qx, qw = vertex_adj_quad(3, 3, 3)
+    out = oracle.pairs_integrate('elasticH3', qx, qw, pts, tris,
+                                 np.array([[0, 1, 0, 0, 0]]), True, [1.0, 0.25])
+    g = np.load(os.path.join(GOLD, 'test_gpu_vert_adjacent.npy'))
+    np.testing.assert_almost_equal(out, g, 6)
+    np.testing.assert_allclose(out, g, rtol=0, atol=1e-6 * np.abs(g).max())
+
+
+def _quad(f, q):
+    return np.sum(f(q[0]) * q[1])
+
+
+def test_quadrature_closed_forms():
+    # tectosaur tests/test_quadrature.py:55-105
+    from tectosaur_amd.quadrature import gaussxw, map_to
+    np.testing.assert_almost_equal(_quad(lambda x: x ** 7, map_to(gaussxw(4), [0, 1])), 1 / 8)
+    np.testing.assert_almost_equal(_quad(lambda x: 1, gauss2d_tri(2)), 0.5)
+    np.testing.assert_almost_equal(
+        _quad(lambda x: x[:, 0] ** 3 * x[:, 1] ** 4, gauss2d_tri(5)), 1 / 2520, 12)
+    np.testing.assert_almost_equal(
+        _quad(lambda x: np.sin(np.exp(x[:, 0] * x[:, 1] * 5)), gauss2d_tri(7)), 0.426659055902, 4)
+    assert gauss2d_tri(3)[0].shape[0] == 7
+    q = gauss4d_tri(3, 3)
+    np.testing.assert_almost_equal(_quad(lambda x: 1, q), 0.25)
+    np.testing.assert_almost_equal(
+        _quad(lambda x: (x[:, 0] * x[:, 1] * x[:, 2] * x[:, 3]) ** 2, q), 1 / 180 ** 2, 10)
+
+
+@pytest.mark.parametrize('rule', ['va', 'co', 'ea'])
+def test_singular_rules_simple(rule):
+    q = {'va': lambda: vertex_adj_quad(8, 8, 8), 'co': lambda: coincident_quad(10),
+         'ea': lambda: edge_adj_quad(6)}[rule]()
+    np.testing.assert_almost_equal(_quad(lambda p: 1.0, q), 0.25, 7)
+    np.testing.assert_almost_equal(_quad(lambda p: p[:, 0] * p[:, 1] * p[:, 2] * p[:, 3], q),
+                                   1 / 576, 7)
+    np.testing.assert_almost_equal(_quad(lambda p: p[:, 0] ** 6 * p[:, 1] * p[:, 3], q),
+                                   1 / 3024, 7)
+    np.testing.assert_almost_equal(_quad(lambda p: p[:, 0] * p[:, 1] * p[:, 3] ** 6, q),
+                                   1 / 1344, 7)
+    np.testing.assert_almost_equal(_quad(lambda p: p[:, 0] * p[:, 2] ** 6 * p[:, 3], q),
+                                   1 / 3024, 7)
+
+
+def test_singular_rules_converge():
+    # tectosaur tests/test_quadrature.py:112-147: 1/r singular integrands agree at nq = 9, 10.
+    # Edge-adjacent: the rule's frame has the source edge traversed opposite to
+    # the observation edge (obs v0 = src v1, obs v1 = src v0, as
+    # resolve_ea_rotation sets pairs up), so the source point is (1 - sx, -sy).
+    # The reference test places it at (sx, -sy), a geometry the rule is not
+    # built for; it converges only to ~1e-3 there.
+    def co(p):
+        return 1 / np.sqrt((p[:, 0] - p[:, 2]) ** 2 + (p[:, 1] - p[:, 3]) ** 2)
+
+    def ea(p):
+        return 1 / np.sqrt((p[:, 0] - 1 + p[:, 2]) ** 2 + (p[:, 1] + p[:, 3]) ** 2)
+
+    def va(p):
+        return 1 / np.sqrt((p[:, 0] + p[:, 2]) ** 2 + (p[:, 1] + p[:, 3]) ** 2)
+
+    for f, rule in ((co, coincident_quad), (ea, edge_adj_quad),
+                    (va, lambda n: vertex_adj_quad(n, n, n))):
+        a, b = _quad(f, rule(9)), _quad(f, rule(10))
+        np.testing.assert_almost_equal(a, b)
+
+
+def test_rule_sizes():
+    assert coincident_quad(8)[0].shape[0] == 24576
+    assert edge_adj_quad(8)[0].shape[0] == 20480
+    assert vertex_adj_quad(8, 8, 8)[0].shape[0] == 8192
+    assert gauss4d_tri(8, 8)[0].shape[0] == 4096
+    assert coincident_quad(6)[0].shape[0] == 6048
+
+
+def test_make_rect_and_sphere_counts():
+    pts, tris = mesh.make_rect(33, 33, [[-1000, 0, 0], [1000, 0, 0], [1000, 0, -2000],
+                                        [-1000, 0, -2000]])
+    assert tris.shape == (2048, 3) and pts.shape == (33 * 33, 3)
+    p, t = mesh.make_icosphere([0, 0, 0], 1.0, 2)
+    assert t.shape[0] == 320 and np.allclose(np.linalg.norm(p, axis=1), 1.0)
+    n = mesh.tri_normals(p, t)
+    c = p[t].mean(axis=1)
+    assert np.all(np.einsum('ij,ij->i', n, c) > 0)  # outward
+    p, t = mesh.make_sphere([0, 0, 0], 1.0, 2)
+    assert t.shape[0] == 128
+
+
+def test_farfield_pts_bruteforce(oracle):
+    # a1: the oracle sum equals an explicit tensor sum on a small random set
+    rng = np.random.default_rng(5)
+    n = 40
+    obs = rng.random((n, 3)); src = rng.random((n, 3)); src[3] = obs[7]  # one r2 == 0 pair
+    on = rng.normal(size=(n, 3)); on /= np.linalg.norm(on, axis=1)[:, None]
+    sn = rng.normal(size=(n, 3)); sn /= np.linalg.norm(sn, axis=1)[:, None]
+    v = rng.normal(size=(n, 3))
+    for K in KS:
+        out = oracle.farfield_pts(K, obs, on, src, sn, v.reshape(-1), [1.0, 0.25]).reshape(n, 3)
+        ref = np.zeros((n, 3))
+        for i in range(n):
+            for j in range(n):
+                D = src[j] - obs[i]
+                if D @ D == 0:
+                    continue
+                ref[i] += oracle.kernel_tensor(K, D, on[i], sn[j], [1.0, 0.25]) @ v[j]
+        np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize('K', KS)
+def test_farfield_tris_equals_dense_pairs(oracle, K):
+    # a2 (matrix-free, vector code) == the dense pair integral (tensor code)
+    # summed against the input, on a small separated mesh pair.
+    m1 = mesh.make_rect(4, 4, [[-1, 0, 1], [-1, 0, -1], [1, 0, -1], [1, 0, 1]])
+    m2 = mesh.make_rect(4, 4, [[-1, 2, 1], [-1, 2, -1], [1, 2, -1], [1, 2, 1]])
+    pts, tris = mesh.concat_two_no_remove_duplicates(m1, m2)
+    n = tris.shape[0]
+    rng = np.random.default_rng(6)
+    v = rng.normal(size=n * 9)
+    qx2, qw2 = gauss2d_tri(2)
+    out = oracle.farfield_tris(K, qx2, qw2, pts, tris, tris, v, [1.0, 0.25])
+    qx4, qw4 = gauss4d_tri(2, 2)
+    pairs = np.array([(i, j) for i in range(n) for j in range(n)])
+    blocks = oracle.pairs_integrate(K, qx4, qw4, pts, tris, pairs, True, [1.0, 0.25])
+    dense = blocks.reshape(n, n, 9, 9).transpose(0, 2, 1, 3).reshape(9 * n, 9 * n)
+    np.testing.assert_allclose(out, dense @ v, rtol=1e-10, atol=1e-10 * np.abs(out).max())
