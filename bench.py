@@ -96,7 +96,7 @@ def main():
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-rows', type=int, default=512)
+    ap.add_argument('--cpu-rows', type=int, default=4096)
     ap.add_argument('--kernels', default='elasticU3,elasticT3,elasticA3,elasticH3')
     args = ap.parse_args()
     if args.gpus != 1:

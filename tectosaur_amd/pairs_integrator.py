@@ -28,6 +28,9 @@ class PairsIntegrator:
         self.float_type = float_type
         self.params = N.params_array(params)
         self.device = N.require_device()
+        tris = np.asarray(tris)
+        if tris.size and (tris.min() < 0 or tris.max() >= np.asarray(pts).shape[0]):
+            raise IndexError('triangle vertex index out of range')
         f64 = dict(dtype=torch.float64, device=self.device)
         self.gpu_pts = torch.as_tensor(np.ascontiguousarray(pts, dtype=np.float64), **f64)
         self.gpu_tris = torch.as_tensor(np.ascontiguousarray(tris, dtype=np.int32),
@@ -66,6 +69,7 @@ class PairsIntegrator:
         n = pairs_list.shape[0]
         if n == 0:
             return np.empty((0, 3, 3, 3, 3), dtype=np.float64)
+        check_pairs(pairs_list, self.gpu_tris.shape[0])
         p = torch.as_tensor(np.ascontiguousarray(pairs_list.astype(np.int32)),
                             device=self.device)
         return self.pairs_quad_dev(q, p, check0).cpu().numpy().reshape((n, 3, 3, 3, 3))
@@ -98,3 +102,17 @@ class PairsIntegrator:
         if p.shape[1] == 4:  # (obs, src, obs_clicks, src_clicks) with no flip column
             p = np.concatenate([p, np.zeros((p.shape[0], 1), dtype=np.int64)], axis=1)
         return p
+
+
+def check_pairs(pairs, n_tris):
+    """Host-side validation before a launch: the kernel trusts these indices."""
+    p = np.asarray(pairs)
+    if p.ndim != 2 or p.shape[1] not in (2, 5):
+        raise ValueError('pairs must be [n, 2] or [n, 5]')
+    if p.shape[0] == 0:
+        return
+    if p[:, :2].min() < 0 or p[:, :2].max() >= n_tris:
+        raise IndexError('pair triangle index out of range [0, %d)' % n_tris)
+    if p.shape[1] == 5 and (p[:, 2:4].min() < 0 or p[:, 2:4].max() > 2 or
+                            p[:, 4].min() < 0 or p[:, 4].max() > 1):
+        raise ValueError('rotation clicks must be 0..2 and flip 0/1')
