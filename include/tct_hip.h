@@ -183,6 +183,21 @@ int tct_find_near_pairs(int64_t n_a, const double *ca, const double *ra, int64_t
                         const double *cb, const double *rb, double threshold, int64_t *out,
                         int64_t cap, int64_t *n_out);
 
+/* ---------------- host-side constraint matrix (CPU) ---------------------
+ * Reference: build_constraint_matrix / reduce_constraints
+ * (tectosaur/fast_constraints.cpp:120-300, tectosaur/constraints.py:10-17).
+ * Constraint i is sum_k term_vals[.] u[term_dofs[.]] = rhs_i with n_terms[i]
+ * terms taken in order.  Writes u = cm x + rhs_mat rhs as COO triplets:
+ * cm (cm_rows, cm_cols, cm_vals; *cm_n entries) and rhs_mat (r_rows, r_cols,
+ * r_vals; *r_n entries); *n_eliminated dofs are removed (cm has
+ * n_total_dofs - *n_eliminated columns).  If cap_cm < *cm_n or cap_r < *r_n
+ * nothing is written and the caller retries with the reported sizes. */
+int tct_build_constraint_matrix(int64_t n_cs, const int64_t *n_terms, const int64_t *term_dofs,
+                                const double *term_vals, int64_t n_total_dofs, int64_t cap_cm,
+                                int64_t *cm_rows, int64_t *cm_cols, double *cm_vals,
+                                int64_t *cm_n, int64_t cap_r, int64_t *r_rows, int64_t *r_cols,
+                                double *r_vals, int64_t *r_n, int64_t *n_eliminated);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,8 @@ SIGNATURES = {
     'tct_bsr9_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_csr_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_find_near_pairs': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _vp, _i64, _vp],
+    'tct_build_constraint_matrix': [_i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
+                                    _vp, _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {'tct_last_error': ctypes.c_char_p}
 

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libtct_hip.so')
 HIP_SOURCES = ['abi.hip', 'far_p2p.hip', 'pairs.hip', 'sparse.hip']
-HOST_SOURCES = ['host_mesh.cpp']
+HOST_SOURCES = ['host_mesh.cpp', 'host_constraints.cpp']
 ARCH = 'gfx950'
 
 

@@ -390,6 +390,7 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
     int gl = lanes_per_pair;
     if (gl == 0) gl = nq >= 1024 ? 64 : (nq >= 128 ? 16 : (nq >= 24 ? 4 : 1));
     TCT_CHECK_ARG(gl == 1 || gl == 4 || gl == 16 || gl == 64, "pairs: lanes_per_pair %d", gl);
+    TCT_CHECK_ARG(n_pairs * gl < (1LL << 31), "pairs: n_pairs * lanes_per_pair >= 2^31");
     const ElasticConsts c = elastic_consts(params_host);
     hipStream_t st = as_stream(stream);
     const int np = (int)n_pairs, q = (int)nq;

@@ -1,0 +1,82 @@
+"""``MassOp``: the P1 Galerkin mass matrix, drop-in for
+tectosaur/ops/mass_op.py:9-42 (with the entries of
+tectosaur/ops/_mass_op.cpp:11-40).  Built once on the host (block diagonal,
+27 entries per triangle for tensor_dim 3), applied on the device by the CSR
+kernel of sparse.hip."""
+import numpy as np
+import scipy.sparse
+import torch
+
+from . import _native as N
+from .quadrature import gauss2d_tri
+
+
+class DeviceCSR:
+    """FP64 CSR matrix on the device (rows < 2^31)."""
+
+    def __init__(self, A):
+        A = scipy.sparse.csr_matrix(A)
+        A.sort_indices()
+        dev = N.require_device()
+        self.device = dev
+        self.shape = A.shape
+        self.indptr = torch.as_tensor(A.indptr.astype(np.int32), device=dev)
+        self.indices = torch.as_tensor(A.indices.astype(np.int32), device=dev)
+        self.data = torch.as_tensor(A.data.astype(np.float64), device=dev)
+
+    def dot_dev(self, x, out=None, accumulate=False):
+        if out is None:
+            out = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+            accumulate = False
+        N.call('tct_csr_spmv', self.shape[0], N.ptr(self.indptr), N.ptr(self.indices),
+               N.ptr(self.data), N.ptr(x), N.ptr(out), int(bool(accumulate)), N.stream_ptr())
+        return out
+
+
+class MassOp:
+    def __init__(self, nq, pts, tris, tensor_dim=3):
+        qx, qw = gauss2d_tri(nq)
+        pts = np.asarray(pts, dtype=np.float64)
+        tris = np.asarray(tris)
+        tri_pts = pts[tris]
+        basis = np.stack([1 - qx[:, 0] - qx[:, 1], qx[:, 0], qx[:, 1]], axis=1)
+        un = np.cross(tri_pts[:, 2] - tri_pts[:, 0], tri_pts[:, 2] - tri_pts[:, 1])
+        jac = np.linalg.norm(un, axis=1)
+        bf = np.einsum('q,qa,qb->ab', qw, basis, basis).reshape(-1)
+        n = tris.shape[0]
+        i = np.arange(n)[:, None, None, None]
+        b1 = np.arange(3)[None, :, None, None]
+        b2 = np.arange(3)[None, None, :, None]
+        d = np.arange(tensor_dim)[None, None, None, :]
+        rows = ((i * 3 + b1) * tensor_dim + d) + 0 * b2
+        cols = ((i * 3 + b2) * tensor_dim + d) + 0 * b1
+        vals = (jac[:, None, None, None] * bf.reshape(3, 3)[None, :, :, None]) + 0 * d
+        n_rows = n * 3 * tensor_dim
+        self.shape = (n_rows, n_rows)
+        self.mat = scipy.sparse.csr_matrix((vals.reshape(-1), (rows.reshape(-1),
+                                                               cols.reshape(-1))),
+                                           shape=self.shape)
+        self._dev = None
+
+    @property
+    def dev(self):
+        if self._dev is None:
+            self._dev = DeviceCSR(self.mat)
+        return self._dev
+
+    def dot(self, v):
+        return self.mat.dot(v)
+
+    def dot_dev(self, x, out=None, accumulate=False):
+        return self.dev.dot_dev(x, out, accumulate)
+
+    def nearfield_dot(self, v):
+        return self.dot(v)
+
+    def nearfield_no_correction_dot(self, v):
+        return self.dot(v)
+
+    def farfield_dot(self, v):
+        shape = [self.shape[0]]
+        shape.extend(v.shape[1:])
+        return np.zeros(shape)
