@@ -90,6 +90,36 @@ def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
         {'f64_vs_oracle_rel_l2': rl2(out64_rows), 'f32_vs_oracle_rel_l2': rl2(out32_rows)}
 
 
+def serve_bench(n_req, max_tris, rate, seed=0, max_active=32):
+    """Served traction -> slip solves: one Poisson stream of fault requests
+    through the continuous-batching mixed-precision server and through the
+    plain FP64 one-at-a-time path; latency = finish - arrival (queueing,
+    device setup and solve)."""
+    import torch
+    from tectosaur_amd.serve import Server, summarize
+    from tectosaur_amd.traction_slip import random_requests
+    warm = random_requests(2, rate=0.0, seed=seed + 99, n_min=512, n_max=1024)
+    Server(mode='mixed', max_active=2).run(warm, realtime=False)
+    Server(mode='f64', max_active=1).run(warm, realtime=False)
+    reqs = random_requests(n_req, rate=rate, seed=seed, n_min=512, n_max=max_tris)
+    out = {'workload': '%d fault requests, Poisson %.2f/s, strike U(0,360) dip U(20,90) '
+                       'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near /'
+                       ' H3 far, tol 1e-8' % (n_req, rate, 512, max_tris)}
+    for name, mode, ma in (('batched_mixed', 'mixed', max_active), ('sequential_f64', 'f64', 1)):
+        srv = Server(mode=mode, max_active=ma)
+        torch.cuda.synchronize()
+        recs = srv.run(reqs, realtime=True)
+        sm = summarize(recs, srv.makespan)
+        sm['ticks'] = srv.ticks
+        out[name] = sm
+        print('# serve %s: %s' % (name, json.dumps(sm)), flush=True)
+    b, f = out['batched_mixed'], out['sequential_f64']
+    out['speedup_solves_per_s'] = b['solves_per_s'] / f['solves_per_s']
+    out['p50_ratio'] = f['p50_s'] / b['p50_s']
+    out['p99_ratio'] = f['p99_s'] / b['p99_s']
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -98,6 +128,10 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-rows', type=int, default=4096)
     ap.add_argument('--kernels', default='elasticU3,elasticT3,elasticA3,elasticH3')
+    ap.add_argument('--serve-requests', type=int, default=48)
+    ap.add_argument('--serve-max-tris', type=int, default=8192)
+    ap.add_argument('--serve-rate', type=float, default=4.0)
+    ap.add_argument('--no-serve', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
@@ -173,6 +207,8 @@ def main():
                      'flops_per_pair': 102},
         'kernels': per_kernel,
     }
+    if not args.no_serve:
+        line['served'] = serve_bench(args.serve_requests, args.serve_max_tris, args.serve_rate)
     if not args.no_cpu_baseline:
         rng = np.random.default_rng(1)
         rows = np.sort(rng.choice(n, size=min(args.cpu_rows, n), replace=False))

@@ -1,0 +1,193 @@
+"""Continuous-batching server for traction -> slip solves on one GPU.
+
+Requests (``traction_slip.FaultRequest``) arrive over time.  The loop
+admits an arrived request as soon as a slot is free and its estimated
+device footprint fits the free-HBM budget, builds its operators on the
+device, and from then on keeps its whole Krylov state in HBM.  Every tick
+advances every active solve by one GMRES step: each solve enqueues its
+device work (matvec, orthogonalisation) on the stream, the small per-solve
+results (Hessenberg column or residual norm) are concatenated on the device
+and cross to the host in one transfer, and the host-side Givens updates run.
+A converged (or failed) solve leaves the batch at once and its slot and
+memory are reused; requests arriving meanwhile join the next tick.
+
+Each solve's arithmetic does not depend on which other solves share its
+ticks (its kernels read only its own buffers), so a result is the same alone
+and in the mix.
+
+``serve_sequential`` is the one-at-a-time reference path (plain FP64 by
+default) the batched path is compared with.
+"""
+import time
+
+import numpy as np
+import torch
+
+from .solver import ConstrainedOperator, make_state
+from .traction_slip import TractionSlipProblem, fault_mesh, request_traction
+
+
+class SolveRecord:
+    def __init__(self, req):
+        self.req = req
+        self.rid = req.rid
+        self.arrival = req.arrival
+        self.admitted = None
+        self.finished = None
+        self.status = None
+        self.iterations = 0
+        self.restarts = 0
+        self.rel_residual = None
+        self.slip = None
+        self.n_tris = None
+        self.pair_evals = 0
+        self.n_matvec = None
+        self.setup_s = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+    def as_dict(self):
+        return dict(rid=self.rid, n_tris=self.n_tris, arrival=self.arrival,
+                    admitted=self.admitted, finished=self.finished, latency=self.latency,
+                    status=self.status, iterations=self.iterations, restarts=self.restarts,
+                    rel_residual=self.rel_residual, setup_s=self.setup_s,
+                    n_matvec=self.n_matvec)
+
+
+def estimate_bytes(n_tris, restart):
+    """Device footprint of one solve: farfield layouts and partial sums,
+    nearfield blocks (corrected and uncorrected, ~45 pairs per triangle),
+    constraint and mass matrices, Krylov basis."""
+    n_pts = 3 * n_tris
+    far = n_pts * (24 + 48 + 96 + 48 + 8 * 3 * 8 + 16)
+    near = 45 * n_tris * (2 * 648 + 16)
+    n_free = int(1.6 * n_tris) + 64
+    krylov = (restart + 5) * n_free * 8
+    return far + near + krylov + 64 * n_tris * 8
+
+
+class _Active:
+    def __init__(self, rec, prob, op, state, nbytes):
+        self.rec = rec
+        self.prob = prob
+        self.op = op
+        self.state = state
+        self.nbytes = nbytes
+
+
+def _build(req, mode, tol, restart, maxiter, orders):
+    pts, tris = fault_mesh(req)
+    t = request_traction(req, tris.shape[0])
+    precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
+    prob = TractionSlipProblem(pts, tris, t, orders=orders, farfield_precisions=precisions)
+    op = ConstrainedOperator(prob.H, prob.cm)
+    b = op.cmT.dot_dev(prob.rhs)
+    state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
+    return prob, op, state
+
+
+def _finalize(a, now, keep_slip):
+    rec, st = a.rec, a.state
+    rec.finished = now
+    rec.status = st.status
+    rec.iterations = st.it
+    rec.restarts = st.restarts
+    rec.rel_residual = st.rel
+    rec.n_matvec = dict(a.op.n_matvec)
+    if keep_slip:
+        rec.slip = a.op.cm.dot_dev(st.x).cpu().numpy()
+
+
+class Server:
+    def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
+                 hbm_fraction=0.8, orders=None, keep_slip=False):
+        self.mode = mode
+        self.max_active = max_active
+        self.tol = tol
+        self.restart = restart
+        self.maxiter = maxiter
+        self.hbm_fraction = hbm_fraction
+        self.orders = orders
+        self.keep_slip = keep_slip
+        self.ticks = 0
+
+    def run(self, requests, realtime=True, log=None):
+        """Serve ``requests`` (sorted by arrival).  With realtime=True a
+        request is admitted no earlier than its arrival time after start;
+        otherwise all are present at t = 0.  Returns SolveRecords."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        recs = {}
+        active = []
+        used = 0
+        free0, _ = torch.cuda.mem_get_info()
+        budget = self.hbm_fraction * free0
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0
+        qi = 0
+        while qi < len(pending) or active:
+            now = clock()
+            # admission
+            while qi < len(pending) and len(active) < self.max_active:
+                req = pending[qi]
+                if realtime and req.arrival > now:
+                    break
+                need = estimate_bytes(req.n_tris, self.restart)
+                if active and used + need > budget:
+                    break
+                rec = SolveRecord(req)
+                if not realtime:
+                    rec.arrival = 0.0
+                rec.admitted = clock()
+                prob, op, state = _build(req, self.mode, self.tol, self.restart, self.maxiter,
+                                         self.orders)
+                rec.n_tris = prob.n_tris
+                rec.pair_evals = prob.pair_evals
+                rec.setup_s = clock() - rec.admitted
+                active.append(_Active(rec, prob, op, state, need))
+                recs[req.rid] = rec
+                used += need
+                qi += 1
+                now = clock()
+            if not active:
+                if qi < len(pending):
+                    time.sleep(max(0.0, min(0.01, pending[qi].arrival - clock())))
+                continue
+            # one tick: every active solve advances one step, one transfer
+            outs = [a.state.issue() for a in active]
+            sizes = [o.shape[0] for o in outs]
+            host = torch.cat(outs).cpu().numpy()
+            self.ticks += 1
+            off = 0
+            still = []
+            now = clock()
+            for a, s in zip(active, sizes):
+                done = a.state.consume(host[off:off + s])
+                off += s
+                if done:
+                    _finalize(a, now, self.keep_slip)
+                    used -= a.nbytes
+                    if log:
+                        log(a.rec)
+                else:
+                    still.append(a)
+            active = still
+        torch.cuda.synchronize()
+        self.makespan = clock()
+        return [recs[r.rid] for r in pending]
+
+
+def serve_sequential(requests, mode='f64', realtime=True, **kw):
+    """The one-at-a-time path: each request solved alone, in arrival order."""
+    return Server(mode=mode, max_active=1, **kw).run(requests, realtime=realtime)
+
+
+def summarize(recs, makespan):
+    lat = np.array([r.latency for r in recs])
+    ok = [r for r in recs if r.status == 'converged']
+    return dict(n=len(recs), converged=len(ok), solves_per_s=len(recs) / makespan,
+                makespan_s=makespan, p50_s=float(np.percentile(lat, 50)),
+                p99_s=float(np.percentile(lat, 99)), mean_s=float(lat.mean()),
+                max_rel_residual=float(max(r.rel_residual for r in recs)),
+                iterations_mean=float(np.mean([r.iterations for r in recs])))

@@ -1,0 +1,283 @@
+"""Device-resident GMRES for constrained BEM systems.
+
+The reference solves cm^T A cm x = cm^T (b - A c_rhs) with
+scipy.sparse.linalg.gmres on the host, one host<->device round trip per
+matvec (tectosaur/simple_solver.py:14-34, examples/solve.py:229-265).  Here
+the Krylov vectors, the operator and the orthogonalisation stay in HBM:
+
+  * ``ConstrainedOperator`` applies cm^T (A (cm x)) with device CSR matvecs
+    around the summed farfield + nearfield matvec;
+  * ``GmresState`` is restarted GMRES(m) with classical Gram-Schmidt
+    repeated twice (CGS2, two GEMVs per pass) in FP64 on the device; only
+    the Hessenberg column (m + 2 doubles) crosses to the host per iteration
+    for the Givens update.  It is a step machine (``issue`` enqueues device
+    work and returns a small device tensor, ``consume`` takes its host
+    value), so a scheduler can advance many solves with one host sync;
+  * mixed precision: the Arnoldi matvecs use the FP32 farfield and every
+    restart recomputes the true residual b - A x with the FP64 farfield, so
+    the restarts are iterative-refinement steps and the converged residual
+    is the FP64 operator's.  A cycle stops once it has reduced the current
+    residual by ``inner_reduction`` (the FP32 operator's accuracy bounds what
+    one cycle can usefully gain).
+
+Tolerances are relative to ||rhs|| of the constrained system, as scipy's.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from .mass_op import DeviceCSR
+
+
+class ConstrainedOperator:
+    """x -> cm^T A (cm x) on the device; A has dot_dev(v, out, prec)."""
+
+    def __init__(self, A, cm):
+        self.A = A
+        self.cm = DeviceCSR(cm)
+        self.cmT = DeviceCSR(cm.T.tocsr())
+        self.device = self.cm.device
+        self.shape = (cm.shape[1], cm.shape[1])
+        self._u = torch.empty(cm.shape[0], dtype=torch.float64, device=self.device)
+        self._y = torch.empty(cm.shape[0], dtype=torch.float64, device=self.device)
+        self.n_matvec = {'f32': 0, 'f64': 0}
+
+    def apply(self, x, out=None, prec='f64'):
+        if out is None:
+            out = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+        self.cm.dot_dev(x, self._u)
+        self.A.dot_dev(self._u, self._y, prec)
+        self.cmT.dot_dev(self._y, out)
+        self.n_matvec[prec] += 1
+        return out
+
+
+def _givens(a, b):
+    if b == 0.0:
+        return 1.0, 0.0
+    r = math.hypot(a, b)
+    return a / r, b / r
+
+
+class GmresResult:
+    def __init__(self, x, converged, iterations, restarts, residuals, rel_residual, seconds,
+                 status):
+        self.x = x
+        self.converged = converged
+        self.iterations = iterations
+        self.restarts = restarts
+        self.residuals = residuals
+        self.rel_residual = rel_residual
+        self.seconds = seconds
+        self.status = status  # 'converged', 'maxiter', 'nonfinite', 'breakdown'
+        self.n_matvec = None
+
+    @property
+    def info(self):
+        return 0 if self.converged else max(self.iterations, 1)
+
+
+class GmresState:
+    """Restarted GMRES as a step machine.
+
+    apply(x, out, prec) computes the operator on device tensors.  Drive it
+    with ``while not s.done: s.consume(s.issue().cpu())`` (what ``run``
+    does), or interleave many states with one transfer per tick."""
+
+    def __init__(self, apply, b, tol=1e-8, restart=200, maxiter=2000, inner_prec='f64',
+                 outer_prec='f64', inner_reduction=None, x0=None, callback=None):
+        self.apply = apply
+        self.b = b
+        dev = b.device
+        n = b.shape[0]
+        self.n = n
+        self.m = max(1, min(restart, n))
+        self.tol = tol
+        self.maxiter = maxiter
+        self.inner_prec = inner_prec
+        self.outer_prec = outer_prec
+        self.inner_reduction = inner_reduction
+        self.callback = callback
+        self.x = torch.zeros(n, dtype=torch.float64, device=dev) if x0 is None else x0.clone()
+        self._fresh = x0 is None
+        self.V = torch.empty((self.m + 1, n), dtype=torch.float64, device=dev)
+        self.w = torch.empty(n, dtype=torch.float64, device=dev)
+        self.r = torch.empty(n, dtype=torch.float64, device=dev)
+        self.it = 0
+        self.restarts = 0
+        self.residuals = []
+        self.rel = float('inf')
+        self.done = False
+        self.status = None
+        self.phase = 'norm_b'
+        self.t0 = time.perf_counter()
+
+    @property
+    def device_bytes(self):
+        return (self.m + 4) * self.n * 8
+
+    # -- device side --------------------------------------------------------
+    def issue(self):
+        """Enqueue the next step's device work; return the device tensor
+        whose host value ``consume`` needs."""
+        if self.phase == 'norm_b':
+            return torch.linalg.vector_norm(self.b).reshape(1)
+        if self.phase == 'residual':
+            if self._fresh and self.restarts == 0:
+                self.r.copy_(self.b)
+            else:
+                self.apply(self.x, self.r, self.outer_prec)
+                torch.sub(self.b, self.r, out=self.r)
+            return torch.linalg.vector_norm(self.r).reshape(1)
+        # arnoldi step j
+        j = self.j
+        self.apply(self.V[j], self.w, self.inner_prec)
+        Vj = self.V[:j + 1]
+        h1 = torch.mv(Vj, self.w)
+        self.w.sub_(torch.mv(Vj.t(), h1))
+        h2 = torch.mv(Vj, self.w)
+        self.w.sub_(torch.mv(Vj.t(), h2))
+        return torch.cat([h1 + h2, torch.linalg.vector_norm(self.w).reshape(1)])
+
+    # -- host side ----------------------------------------------------------
+    def consume(self, vals):
+        vals = np.asarray(vals, dtype=np.float64).reshape(-1)
+        if self.phase == 'norm_b':
+            self.bnorm = float(vals[0])
+            if self.bnorm == 0.0 or not math.isfinite(self.bnorm):
+                self.x.zero_()
+                ok = self.bnorm == 0.0
+                self.rel = 0.0 if ok else float('nan')
+                self.residuals.append(self.rel)
+                return self._finish('converged' if ok else 'nonfinite')
+            self.target = self.tol * self.bnorm
+            self.phase = 'residual'
+            return False
+        if self.phase == 'residual':
+            beta = float(vals[0])
+            self.rel = beta / self.bnorm
+            self.residuals.append(self.rel)
+            if self.callback is not None:
+                self.callback(self.rel)
+            if not math.isfinite(beta):
+                return self._finish('nonfinite')
+            if beta <= self.target:
+                return self._finish('converged')
+            if self.it >= self.maxiter:
+                return self._finish('maxiter')
+            self.restarts += 1
+            self.V[0].copy_(self.r).div_(beta)
+            m = self.m
+            self.H = np.zeros((m + 1, m))
+            self.g = np.zeros(m + 1)
+            self.g[0] = beta
+            self.cs = np.zeros(m)
+            self.sn = np.zeros(m)
+            self.cycle_target = self.target
+            if self.inner_reduction is not None:
+                self.cycle_target = max(self.target * 0.5, self.inner_reduction * beta)
+            self.j = 0
+            self.phase = 'arnoldi'
+            return False
+        # arnoldi
+        j = self.j
+        col = vals.copy()
+        hnext = float(col[j + 1])
+        self.it += 1
+        cs, sn = self.cs, self.sn
+        for i in range(j):
+            t = cs[i] * col[i] + sn[i] * col[i + 1]
+            col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1]
+            col[i] = t
+        cs[j], sn[j] = _givens(col[j], col[j + 1])
+        col[j] = cs[j] * col[j] + sn[j] * col[j + 1]
+        col[j + 1] = 0.0
+        self.H[:j + 2, j] = col
+        self.g[j + 1] = -sn[j] * self.g[j]
+        self.g[j] = cs[j] * self.g[j]
+        est = abs(self.g[j + 1])
+        k = j + 1
+        end = (not math.isfinite(est) or est <= self.cycle_target or hnext == 0.0 or
+               k >= self.m or self.it >= self.maxiter)
+        if not end:
+            self.V[k].copy_(self.w).div_(hnext)
+            self.j = k
+            return False
+        # close the cycle: x += V_k y
+        if math.isfinite(est):
+            y = np.linalg.solve(np.triu(self.H[:k, :k]), self.g[:k])
+            if np.all(np.isfinite(y)):
+                self.x.add_(torch.mv(self.V[:k].t(),
+                                     torch.as_tensor(y, dtype=torch.float64, device=self.x.device)))
+        self.phase = 'residual'
+        return False
+
+    def _finish(self, status):
+        self.done = True
+        self.status = status
+        return True
+
+    def result(self):
+        return GmresResult(self.x, self.status == 'converged', self.it, self.restarts,
+                           self.residuals, self.rel, time.perf_counter() - self.t0, self.status)
+
+    def run(self):
+        while not self.done:
+            self.consume(self.issue().cpu().numpy())
+        return self.result()
+
+
+def gmres(apply, b, x0=None, tol=1e-8, restart=200, maxiter=2000, inner_prec='f64',
+          outer_prec='f64', inner_reduction=None, callback=None):
+    """Restarted GMRES on the device (see GmresState)."""
+    return GmresState(apply, b, tol=tol, restart=restart, maxiter=maxiter, inner_prec=inner_prec,
+                      outer_prec=outer_prec, inner_reduction=inner_reduction, x0=x0,
+                      callback=callback).run()
+
+
+MODES = {
+    # name: (inner_prec, outer_prec, inner_reduction)
+    'f64': ('f64', 'f64', None),
+    'mixed': ('f32', 'f64', 1e-4),
+}
+
+
+def make_state(op, b, mode='mixed', tol=1e-8, restart=200, maxiter=2000, callback=None):
+    inner, outer, red = MODES[mode]
+    return GmresState(op.apply, b, tol=tol, restart=restart, maxiter=maxiter, inner_prec=inner,
+                      outer_prec=outer, inner_reduction=red, callback=callback)
+
+
+def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxiter=6000,
+                      callback=None):
+    """Solve cm^T A cm x = cm^T rhs; return (u = cm x (device), GmresResult).
+
+    mode 'f64': the plain FP64 path (FP64 farfield everywhere).
+    mode 'mixed': FP32 farfield in the Arnoldi cycles, FP64 residual at
+    every restart (GMRES-based iterative refinement)."""
+    if mode not in MODES:
+        raise ValueError('mode must be one of %s' % sorted(MODES))
+    op = ConstrainedOperator(A, cm)
+    b = op.cmT.dot_dev(rhs_dev)
+    res = make_state(op, b, mode, tol, restart, maxiter, callback).run()
+    u = op.cm.dot_dev(res.x)
+    res.n_matvec = dict(op.n_matvec)
+    return u, res
+
+
+def iterative_solve(iop, constraints, rhs=None, tol=1e-8, mode='mixed', restart=1500):
+    """Host-array API of examples/solve.py:229-265: constraints -> cm, solve
+    cm^T (rhs - A c_rhs) and return (cm x + c_rhs as numpy, GmresResult)."""
+    from .constraints import build_constraint_matrix
+    cm, c_rhs, _ = build_constraint_matrix(constraints, iop.shape[1])
+    dev = iop.device
+    c_rhs_d = torch.as_tensor(c_rhs, dtype=torch.float64, device=dev)
+    Ac = iop.dot_dev(c_rhs_d, None, 'f64')
+    if rhs is None:
+        r = -Ac
+    else:
+        r = torch.as_tensor(np.asarray(rhs, dtype=np.float64), device=dev) - Ac
+    u, res = solve_constrained(iop, cm, r, tol=tol, mode=mode, restart=restart)
+    return (u + c_rhs_d).cpu().numpy(), res

@@ -106,3 +106,18 @@ def test_mass_op_integrates_constants():
     M3 = MassOp(3, pts, tris)
     ones = np.ones(M3.shape[1])
     np.testing.assert_allclose(M3.dot(ones).sum(), 3 * 4.0, rtol=1e-14)
+
+
+def test_continuity_cm_equals_general_builder():
+    # the direct vertex-grouping cm equals build_constraint_matrix on
+    # continuity + zero free-edge constraints
+    from tectosaur_amd.traction_slip import continuity_cm, fault_mesh, FaultRequest
+    meshes = [rect(6), mesh.make_icosphere([0, 0, 0], 1.0, 1),
+              fault_mesh(FaultRequest(0, 30.0, 60.0, 1e4, 2.0, 300, np.zeros(3), 0))]
+    for pts, tris in meshes:
+        cs = continuity_constraints(pts, tris, tris.shape[0]) + free_edge_constraints(tris)
+        cm, rhs, _ = build_constraint_matrix(cs, tris.shape[0] * 9)
+        fast = continuity_cm(tris)
+        assert fast.shape == cm.shape
+        assert abs(fast - cm).max() == 0
+        np.testing.assert_array_equal(rhs, 0)

@@ -1,0 +1,95 @@
+"""Traction -> slip solves on the device: converged residual checked under
+the FP64 oracle operator, mixed vs FP64 agreement, and the server's
+alone-vs-in-the-mix identity and failure reporting."""
+import numpy as np
+import pytest
+import torch
+
+from tectosaur_amd.quadrature import gauss2d_tri, gauss4d_tri
+from tectosaur_amd import triangle_rules
+from tectosaur_amd.traction_slip import (FaultRequest, TractionSlipProblem, fault_mesh,
+                                         request_traction)
+
+pytestmark = pytest.mark.gpu
+ORD = dict(nq_coincident=4, nq_edge_adj=4, nq_vert_adjacent=3, nq_far=1, nq_near=3,
+           near_threshold=2.5, quad_mass_order=3)
+
+
+def oracle_apply(oracle, pts, tris, u, nu=0.25):
+    o = ORD
+    rules = {'co': triangle_rules.coincident_quad(o['nq_coincident']),
+             'ea': triangle_rules.edge_adj_quad(o['nq_edge_adj']),
+             'va': triangle_rules.vertex_adj_quad(*(o['nq_vert_adjacent'],) * 3),
+             'near': gauss4d_tri(o['nq_near'], o['nq_near']),
+             'far': gauss4d_tri(o['nq_far'], o['nq_far'])}
+    blocks = oracle.nearfield_blocks(pts, tris, o['nq_coincident'], o['nq_edge_adj'],
+                                     o['nq_vert_adjacent'], o['nq_far'], o['nq_near'],
+                                     o['near_threshold'], 'elasticRH3', 'elasticH3', [1.0, nu],
+                                     rules)
+    y = np.zeros(tris.shape[0] * 9)
+    for prs, corr, _ in blocks.values():
+        y += oracle.bcoo_matvec(prs[:, 0], prs[:, 1], corr, u, (y.shape[0], y.shape[0]))
+    qx, qw = gauss2d_tri(o['nq_far'])
+    y += oracle.farfield_tris('elasticH3', qx, qw, pts, tris, tris, u, [1.0, nu])
+    return y
+
+
+def small_problem(n_tris=200, seed=5):
+    req = FaultRequest(0, 35.0, 55.0, 8000.0, 1.5, n_tris, np.array([0.3, -0.7, 0.2]), seed)
+    pts, tris = fault_mesh(req)
+    return pts, tris, request_traction(req, tris.shape[0])
+
+
+@pytest.mark.parametrize('mode', ['f64', 'mixed'])
+def test_solve_residual_under_oracle_operator(oracle, mode):
+    from tectosaur_amd.solver import solve_constrained
+    pts, tris, t = small_problem()
+    prob = TractionSlipProblem(pts, tris, t, orders=ORD)
+    u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=1e-8, mode=mode, restart=60)
+    assert res.converged and res.rel_residual <= 1e-8
+    u = u.cpu().numpy()
+    b = prob.cm.T @ prob.rhs.cpu().numpy()
+    r = b - prob.cm.T @ oracle_apply(oracle, pts, tris, u)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.1e-8
+    if mode == 'mixed':
+        assert res.n_matvec['f32'] > res.n_matvec['f64']
+
+
+def test_mixed_matches_f64_solution():
+    from tectosaur_amd.solver import solve_constrained
+    pts, tris, t = small_problem(300, 7)
+    prob = TractionSlipProblem(pts, tris, t, orders=ORD)
+    u64, r64 = solve_constrained(prob.H, prob.cm, prob.rhs, tol=1e-10, mode='f64', restart=80)
+    u32, r32 = solve_constrained(prob.H, prob.cm, prob.rhs, tol=1e-10, mode='mixed', restart=80)
+    assert r64.converged and r32.converged
+    u64 = u64.cpu().numpy(); u32 = u32.cpu().numpy()
+    assert np.linalg.norm(u32 - u64) <= 1e-8 * np.linalg.norm(u64)
+
+
+def test_server_result_same_alone_and_in_mix():
+    from tectosaur_amd.serve import Server
+    from tectosaur_amd.traction_slip import random_requests
+    reqs = random_requests(5, rate=0.0, seed=3, n_min=256, n_max=900)
+    mix = Server(mode='mixed', max_active=5, restart=60, orders=ORD, keep_slip=True)
+    recs_mix = mix.run(reqs, realtime=False)
+    assert mix.ticks < sum(r.iterations for r in recs_mix)  # solves shared ticks
+    for req, rm in zip(reqs, recs_mix):
+        alone = Server(mode='mixed', max_active=1, restart=60, orders=ORD, keep_slip=True)
+        ra = alone.run([req], realtime=False)[0]
+        assert rm.status == ra.status == 'converged'
+        assert rm.iterations == ra.iterations
+        # identical kernels on identical buffers: agreement to roundoff (bitwise in practice)
+        assert np.abs(rm.slip - ra.slip).max() <= 1e-12 * np.abs(ra.slip).max()
+
+
+def test_server_reports_failures():
+    from tectosaur_amd.serve import Server
+    from tectosaur_amd.traction_slip import random_requests
+    reqs = random_requests(2, rate=0.0, seed=4, n_min=256, n_max=300)
+    s = Server(mode='mixed', max_active=2, restart=30, maxiter=1, tol=1e-14, orders=ORD)
+    recs = s.run(reqs, realtime=False)
+    assert all(r.status == 'maxiter' for r in recs)
+    bad = random_requests(1, rate=0.0, seed=5, n_min=256, n_max=300)
+    bad[0].traction = np.array([np.nan, 0.0, 0.0])
+    r = Server(mode='mixed', max_active=1, orders=ORD).run(bad, realtime=False)[0]
+    assert r.status == 'nonfinite'

@@ -1,0 +1,66 @@
+"""GMRES (algorithm on CPU torch tensors; the device path is exercised by
+tests/test_solve_gpu.py)."""
+import numpy as np
+import pytest
+import torch
+
+from tectosaur_amd.solver import gmres
+
+
+def dense_apply(A):
+    At = torch.as_tensor(A)
+
+    def apply(x, out, prec):
+        torch.mv(At, x, out=out)
+        return out
+    return apply
+
+
+@pytest.mark.parametrize('restart', [5, 20, 200])
+def test_gmres_matches_direct_solve(restart):
+    rng = np.random.default_rng(1)
+    n = 60
+    A = np.eye(n) * 4 + rng.normal(size=(n, n)) / np.sqrt(n)
+    b = rng.normal(size=n)
+    res = gmres(dense_apply(A), torch.as_tensor(b), tol=1e-12, restart=restart, maxiter=2000)
+    assert res.converged
+    x = res.x.numpy()
+    assert np.linalg.norm(A @ x - b) <= 1e-11 * np.linalg.norm(b)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-9, atol=1e-10)
+
+
+def test_mixed_precision_refinement_reaches_fp64_residual():
+    # inner operator perturbed by 1e-6 (as an FP32 farfield is); restarts use
+    # the exact operator, so the final residual is the exact operator's.
+    rng = np.random.default_rng(2)
+    n = 80
+    A = np.eye(n) * 3 + rng.normal(size=(n, n)) / np.sqrt(n)
+    E = 1e-6 * rng.normal(size=(n, n)) / np.sqrt(n)
+    At, Ait = torch.as_tensor(A), torch.as_tensor(A + E)
+
+    def apply(x, out, prec):
+        torch.mv(At if prec == 'f64' else Ait, x, out=out)
+        return out
+    b = torch.as_tensor(rng.normal(size=n))
+    res = gmres(apply, b, tol=1e-12, restart=50, inner_prec='f32', outer_prec='f64',
+                inner_reduction=1e-4)
+    assert res.converged and res.restarts >= 2
+    assert np.linalg.norm(A @ res.x.numpy() - b.numpy()) <= 1e-12 * np.linalg.norm(b.numpy())
+
+
+def test_gmres_zero_rhs_and_nan():
+    A = np.eye(4)
+    r = gmres(dense_apply(A), torch.zeros(4, dtype=torch.float64), tol=1e-8)
+    assert r.converged and np.all(r.x.numpy() == 0)
+    b = torch.tensor([1.0, np.nan, 0, 0], dtype=torch.float64)
+    r = gmres(dense_apply(A), b, tol=1e-8)
+    assert not r.converged and r.info != 0
+
+
+def test_gmres_reports_nonconvergence():
+    rng = np.random.default_rng(3)
+    n = 50
+    A = rng.normal(size=(n, n))
+    b = torch.as_tensor(rng.normal(size=n))
+    r = gmres(dense_apply(A), b, tol=1e-14, restart=3, maxiter=1)
+    assert not r.converged and r.info != 0 and r.iterations == 1
