@@ -88,6 +88,12 @@ int tct_far_pack_obs(int64_t n, const double *pts, const double *ns, int64_t n_p
 int tct_far_pack_src(int64_t n, const double *pts, const double *ns, int64_t n_pad,
                      float *rec32, double *org, double *rec64, void *stream);
 
+/* tct_far_pack_src_ex: as tct_far_pack_src with rec_words (12, or 24 for the
+ * elasticRH3 record) words per source record; words past the normal are
+ * zeroed. */
+int tct_far_pack_src_ex(int64_t n, const double *pts, const double *ns, int64_t n_pad,
+                        int rec_words, float *rec32, double *org, double *rec64, void *stream);
+
 /* ---------------- farfield: per-matvec strengths -------------------------
  * tct_far_strength_pts: point strengths vec [n][3] f64 (the "input" of
  * farfield_pts{K}, tectosaur/farfield_direct.cl:70-72) -> record strength
@@ -106,6 +112,17 @@ int tct_far_gather_tris(int kernel, int prec, int64_t n_pts, int nq, const int32
                         const double *qphi, const double *wj, const double *v,
                         const double *params_host, void *rec, void *stream);
 
+/* tct_far_gather_curl: the surface-curl kernels RT, RA, RH (farfield_tris_to_tris
+ * with the vector codes of tectosaur/kernels/kernels.cl:100-206).  Per source
+ * point it writes what the point-pair form needs from the source triangle's
+ * surface curl S[d][I] = sum_b curl[b][I] v[9t+3b+d] and the interpolated
+ * input, scaled by w_q J_t.  tri_curl [n_src_tris][3][3] (basis, component)
+ * holds the basis surface curls of each DOF-space source triangle
+ * (integral_primitives.cl:95-112). */
+int tct_far_gather_curl(int kernel, int prec, int64_t n_pts, int nq, const int32_t *tri_of_pt,
+                        const double *qphi, const double *wj, const double *tri_curl,
+                        const double *v, const double *params_host, void *rec, void *stream);
+
 /* ---------------- farfield: the N-body kernel ---------------------------
  * Reference: farfield_pts{K} (tectosaur/farfield_direct.cl:15-113), the
  * hot loop of farfield_tris_to_tris{K} (tectosaur/matrix_free.cl:47-101).
@@ -116,7 +133,9 @@ int tct_far_gather_tris(int kernel, int prec, int64_t n_pts, int nq, const int32
  * r2 <= eps2 are skipped, where eps2 bounds the rounding of one point stored
  * relative to two different group origins (see the packing).  TCT_F64: FP64
  * throughout, absolute coordinates, exact r2 == 0 test (eps2 ignored).
- * Kernels U, T, A, H. */
+ * Kernels U, T, A, H (3 sums per point) and the surface-curl forms RT (3),
+ * RA and RH (9 moments per point, applied to the observation curls by
+ * tct_far_scatter_curl); out_partial is [src_chunks][3 or 9][n_obs_pad]. */
 int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float *loc32,
                 const double *obs_org, const double *obs_soa64, int64_t n_src_pad,
                 const void *rec, const double *src_org, const double *params_host,
@@ -133,6 +152,14 @@ int tct_far_reduce_pts(int64_t n, int64_t n_obs_pad, int chunks, const double *p
 int tct_far_scatter_tris(int64_t n_tris, int nq, const int32_t *pt_of_tri, const double *qphi,
                          const double *wj_obs, int64_t n_obs_pad, int chunks,
                          const double *partial, double *out, void *stream);
+
+/* Galerkin scatter of the 9-moment kernels RA and RH: out[9t+3b+k] from the
+ * per-point moments and the observation triangle's basis surface curls
+ * tri_curl [n_obs_tris][3][3]. */
+int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const int32_t *pt_of_tri,
+                         const double *qphi, const double *wj_obs, const double *tri_curl,
+                         int64_t n_obs_pad, int chunks, const double *partial,
+                         const double *params_host, double *out, void *stream);
 
 /* ---------------- nearfield pair integrator (FP64) ----------------------
  * Reference: single_pairs{Z,N} / single_pairsN_adj (tectosaur/assemble.cl

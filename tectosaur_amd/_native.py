@@ -44,6 +44,10 @@ SIGNATURES = {
     'tct_abi_version': [],
     'tct_far_pack_obs': [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     'tct_far_pack_src': [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    'tct_far_pack_src_ex': [_i64, _vp, _vp, _i64, _int, _vp, _vp, _vp, _vp],
+    'tct_far_gather_curl': [_int, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    'tct_far_scatter_curl': [_int, _i64, _int, _vp, _vp, _vp, _vp, _i64, _int, _vp, _vp, _vp,
+                             _vp],
     'tct_far_strength_pts': [_int, _int, _i64, _vp, _vp, _vp, _vp],
     'tct_far_gather_tris': [_int, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'tct_far_p2p': [_int, _int, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp],

@@ -49,20 +49,33 @@ __device__ __forceinline__ double rsq_masked(double r2, double eps2) {
     return r2 > eps2 ? v : 0.0;
 }
 
-// One (observation point, source point) pair, accumulate into acc[3].
+template <int KID>
+constexpr int rec_words() { return KID == TCT_RH ? 24 : 12; }
+template <int KID>
+constexpr int n_out() { return (KID == TCT_RA || KID == TCT_RH) ? 9 : 3; }
+template <int KID>
+constexpr bool needs_obs_normal() { return KID == TCT_A || KID == TCT_H || KID == TCT_RA; }
+
+// One (observation point, source point) pair, accumulate into acc[n_out].
 // x: observation point (already shifted into the source tile frame);
-// m: observation unit normal; y, n, s: source point, normal, scaled strength;
-// e0, e1: per-source precomputed terms.
+// m: observation unit normal; q: the source record (wave-uniform).
+//   U/T/A/H: q = [y n s e0 e1 -], s the strength scaled by the kernel
+//            constant, e0/e1 per-source dot products;
+//   RT:      q = [y n s e], s = CsRT2 w J in(q), e = CsRT1 w J a with
+//            a_t = sum_{d,I} eps_{I t d} S_{dI} (S the source surface curl);
+//   RA:      q = [y - s -], s = w J in(q);
+//   RH:      q = [y G(9) P(9) -], G = w J CsRH0 S,
+//            P = w J CsRH0 (CsRH1 tr(S) I + CsRH2 S^T - 2 S).
 template <int KID, typename T>
-__device__ __forceinline__ void pair_eval(const T x[3], const T m[3], T y0, T y1, T y2, T n0,
-                                          T n1, T n2, T s0, T s1, T s2, T e0, T e1,
-                                          const KConst<T> &k, T acc[3]) {
-    const T D0 = y0 - x[0], D1 = y1 - x[1], D2 = y2 - x[2];
+__device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *__restrict__ q,
+                                          const KConst<T> &k, T *acc) {
+    const T D0 = q[0] - x[0], D1 = q[1] - x[1], D2 = q[2] - x[2];
     const T r2 = D0 * D0 + D1 * D1 + D2 * D2;
     const T invr = rsq_masked(r2, k.eps2);
     const T w = invr * invr;
     if constexpr (KID == TCT_U) {
         // U: (3-4nu)/r s + (D.s) D / r^3, strengths scaled by CsU1.
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
         const T ds = D0 * s0 + D1 * s1 + D2 * s2;
         const T a = k.c0 * invr;
         const T b = invr * w * ds;
@@ -73,18 +86,21 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], T y0, T y1
         // T: A(-(n.D) s + (n x D) x s) - C (n.D)(D.s) D with
         // (n x D) x s = D (n.s) - n (D.s); strengths scaled by CsT1,
         // k.c0 = CsT0 / CsT1, e0 = n.s.
+        const T n0 = q[3], n1 = q[4], n2 = q[5];
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
         const T q3 = invr * w;
         const T al = k.c0 * q3;
         const T rn = n0 * D0 + n1 * D1 + n2 * D2;
         const T ds = D0 * s0 + D1 * s1 + D2 * s2;
         const T be = -al * rn;
         const T ga = -al * ds;
-        const T de = al * e0 - q3 * w * rn * ds;
+        const T de = al * q[9] - q3 * w * rn * ds;
         acc[0] += be * s0 + ga * n0 + de * D0;
         acc[1] += be * s1 + ga * n1 + de * D1;
         acc[2] += be * s2 + ga * n2 + de * D2;
     } else if constexpr (KID == TCT_A) {
         // A: the adjoint form with the observation normal m.
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
         const T q3 = invr * w;
         const T al = k.c0 * q3;
         const T rn = m[0] * D0 + m[1] * D1 + m[2] * D2;
@@ -96,9 +112,11 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], T y0, T y1
         acc[0] += be * s0 + ga * m[0] + de * D0;
         acc[1] += be * s1 + ga * m[1] + de * D1;
         acc[2] += be * s2 + ga * m[2] + de * D2;
-    } else {
+    } else if constexpr (KID == TCT_H) {
         // H with strengths scaled by CsH0; k = (CsH1, CsH3, 3 CsH1, nu, 3 nu),
         // e0 = CsH2 (n.s), e1 = 3 CsH1 (n.s).
+        const T n0 = q[3], n1 = q[4], n2 = q[5];
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
         const T Q = invr * w;
         const T a = n0 * D0 + n1 * D1 + n2 * D2;
         const T b = m[0] * D0 + m[1] * D1 + m[2] * D2;
@@ -107,29 +125,68 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], T y0, T y1
         const T mn = m[0] * n0 + m[1] * n1 + m[2] * n2;
         const T wa = w * a;
         const T wbc = w * b * c;
-        const T MT = Q * (e0 + k.c2 * wa * c);
+        const T MT = Q * (q[9] + k.c2 * wa * c);
         const T NT = Q * (k.c0 * sm + k.c1 * wbc);
         const T inner = k.c3 * sm - T(5) * wbc;
-        const T DT = Q * w * (e1 * b + k.c1 * c * mn + T(3) * a * inner);
+        const T DT = Q * w * (q[10] * b + k.c1 * c * mn + T(3) * a * inner);
         const T ST = Q * (k.c4 * wa * b + k.c0 * mn);
         acc[0] += n0 * NT + m[0] * MT + D0 * DT + s0 * ST;
         acc[1] += n1 * NT + m[1] * MT + D1 * DT + s1 * ST;
         acc[2] += n2 * NT + m[2] * MT + D2 * DT + s2 * ST;
+    } else if constexpr (KID == TCT_RT) {
+        // RT: invr e + (CsRT0/CsRT1) invr^3 (D.e) D + invr^3 (n.D) s
+        const T i3 = invr * w;
+        const T De = D0 * q[9] + D1 * q[10] + D2 * q[11];
+        const T nD = D0 * q[3] + D1 * q[4] + D2 * q[5];
+        const T cd = k.c0 * i3 * De;
+        const T cs = i3 * nD;
+        acc[0] += invr * q[9] + cd * D0 + cs * q[6];
+        acc[1] += invr * q[10] + cd * D1 + cs * q[7];
+        acc[2] += invr * q[11] + cd * D2 + cs * q[8];
+    } else if constexpr (KID == TCT_RA) {
+        // RA moments: v1 += s / r, v2 += (D.s) D / r^3, v3 += (m.D) s / r^3;
+        // the constants and the observation curls are applied by the scatter.
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
+        const T i3 = invr * w;
+        const T Ds = i3 * (D0 * s0 + D1 * s1 + D2 * s2);
+        const T mD = i3 * (D0 * m[0] + D1 * m[1] + D2 * m[2]);
+        acc[0] += invr * s0;
+        acc[1] += invr * s1;
+        acc[2] += invr * s2;
+        acc[3] += Ds * D0;
+        acc[4] += Ds * D1;
+        acc[5] += Ds * D2;
+        acc[6] += mD * s0;
+        acc[7] += mD * s1;
+        acc[8] += mD * s2;
+    } else {
+        // RH: M[k][m] += P[k][m] / r - 2 D_k (D^T G)_m / r^3
+        const T Dk[3] = {D0, D1, D2};
+        const T qd = T(-2) * invr * w;
+        T u[3];
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) u[mm] = D0 * q[3 + mm] + D1 * q[6 + mm] + D2 * q[9 + mm];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const T dq = qd * Dk[kk];
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+                acc[kk * 3 + mm] += invr * q[12 + kk * 3 + mm] + dq * u[mm];
+        }
     }
 }
 
-template <int KID>
-constexpr bool needs_obs_normal() { return KID == TCT_A || KID == TCT_H; }
-
 // ---------------------------------------------------------------------------
 // FP32 N-body.  grid = (ceil(n_obs_pad / (blockDim*P)), chunks).
-// loc32 [6][n_obs_pad]; obs_org [n_obs_pad/G][3]; rec [n_src_pad][R];
-// src_org [n_src_pad/G][3]; partial [chunks][3][n_obs_pad].
+// loc32 [6][n_obs_pad]; obs_org [n_obs_pad/G][3]; rec [n_src_pad][RW];
+// src_org [n_src_pad/G][3]; partial [chunks][NO][n_obs_pad].
 template <int KID, int P>
 __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
     int n_obs_pad, const float *__restrict__ loc32, const double *__restrict__ obs_org,
     int n_tiles, const float *__restrict__ rec, const double *__restrict__ src_org,
     KConst<float> k, double *__restrict__ partial) {
+    constexpr int RW = rec_words<KID>();
+    constexpr int NO = n_out<KID>();
     const int bd = blockDim.x;
     const int base = blockIdx.x * bd * P;
     const int chunks = gridDim.y;
@@ -138,7 +195,7 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
     const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
 
     float xl[P][3], m[P][3];
-    double oo[P][3], acc64[P][3];
+    double oo[P][3], acc64[P][NO];
     int idx[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -151,53 +208,47 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
             xl[p][d] = loc32[(size_t)d * n_obs_pad + ic];
             m[p][d] = needs_obs_normal<KID>() ? loc32[(size_t)(3 + d) * n_obs_pad + ic] : 0.0f;
             oo[p][d] = obs_org[3 * g + d];
-            acc64[p][d] = 0.0;
         }
+#pragma unroll
+        for (int a = 0; a < NO; ++a) acc64[p][a] = 0.0;
     }
 
     for (int t = t0; t < t1; ++t) {
         const double os0 = src_org[3 * t], os1 = src_org[3 * t + 1], os2 = src_org[3 * t + 2];
-        float xs[P][3], acc[P][3];
+        float xs[P][3], acc[P][NO];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             xs[p][0] = xl[p][0] - (float)(os0 - oo[p][0]);
             xs[p][1] = xl[p][1] - (float)(os1 - oo[p][1]);
             xs[p][2] = xl[p][2] - (float)(os2 - oo[p][2]);
-            acc[p][0] = acc[p][1] = acc[p][2] = 0.0f;
+#pragma unroll
+            for (int a = 0; a < NO; ++a) acc[p][a] = 0.0f;
         }
-        const float *r = rec + (size_t)t * G * R;
+        const float *r = rec + (size_t)t * G * RW;
 #pragma unroll 4
         for (int j = 0; j < G; ++j) {
-            const float *q = r + j * R;
-            const float y0 = q[0], y1 = q[1], y2 = q[2];
-            const float n0 = q[3], n1 = q[4], n2 = q[5];
-            const float s0 = q[6], s1 = q[7], s2 = q[8];
-            const float e0 = q[9], e1 = q[10];
+            const float *q = r + j * RW;
 #pragma unroll
-            for (int p = 0; p < P; ++p)
-                pair_eval<KID, float>(xs[p], m[p], y0, y1, y2, n0, n1, n2, s0, s1, s2, e0, e1, k,
-                                      acc[p]);
+            for (int p = 0; p < P; ++p) pair_eval<KID, float>(xs[p], m[p], q, k, acc[p]);
         }
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            acc64[p][0] += (double)acc[p][0];
-            acc64[p][1] += (double)acc[p][1];
-            acc64[p][2] += (double)acc[p][2];
-        }
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int a = 0; a < NO; ++a) acc64[p][a] += (double)acc[p][a];
     }
 
-    double *out = partial + (size_t)c * 3 * n_obs_pad;
+    double *out = partial + (size_t)c * NO * n_obs_pad;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         if (idx[p] < n_obs_pad) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) out[(size_t)d * n_obs_pad + idx[p]] = acc64[p][d];
+            for (int a = 0; a < NO; ++a) out[(size_t)a * n_obs_pad + idx[p]] = acc64[p][a];
         }
     }
 }
 
 // FP64 N-body ("plain FP64" path): absolute coordinates, FP64 throughout.
-// soa64 [6][n_obs_pad]; rec64 [n_src_pad][R].
+// soa64 [6][n_obs_pad]; rec64 [n_src_pad][RW].
 template <int KID, int P>
 __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
                                                           const double *__restrict__ soa64,
@@ -205,6 +256,8 @@ __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
                                                           const double *__restrict__ rec,
                                                           KConst<double> k,
                                                           double *__restrict__ partial) {
+    constexpr int RW = rec_words<KID>();
+    constexpr int NO = n_out<KID>();
     const int bd = blockDim.x;
     const int base = blockIdx.x * bd * P;
     const int chunks = gridDim.y;
@@ -212,7 +265,7 @@ __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
     const int t0 = (int)(((long)n_tiles * c) / chunks);
     const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
 
-    double x[P][3], m[P][3], acc[P][3];
+    double x[P][3], m[P][3], acc[P][NO];
     int idx[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -223,29 +276,24 @@ __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
         for (int d = 0; d < 3; ++d) {
             x[p][d] = soa64[(size_t)d * n_obs_pad + ic];
             m[p][d] = needs_obs_normal<KID>() ? soa64[(size_t)(3 + d) * n_obs_pad + ic] : 0.0;
-            acc[p][d] = 0.0;
         }
+#pragma unroll
+        for (int a = 0; a < NO; ++a) acc[p][a] = 0.0;
     }
-    const double *r = rec + (size_t)t0 * G * R;
+    const double *r = rec + (size_t)t0 * G * RW;
     const int nj = (t1 - t0) * G;
 #pragma unroll 2
     for (int j = 0; j < nj; ++j) {
-        const double *q = r + (size_t)j * R;
-        const double y0 = q[0], y1 = q[1], y2 = q[2];
-        const double n0 = q[3], n1 = q[4], n2 = q[5];
-        const double s0 = q[6], s1 = q[7], s2 = q[8];
-        const double e0 = q[9], e1 = q[10];
+        const double *q = r + (size_t)j * RW;
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-            pair_eval<KID, double>(x[p], m[p], y0, y1, y2, n0, n1, n2, s0, s1, s2, e0, e1, k,
-                                   acc[p]);
+        for (int p = 0; p < P; ++p) pair_eval<KID, double>(x[p], m[p], q, k, acc[p]);
     }
-    double *out = partial + (size_t)c * 3 * n_obs_pad;
+    double *out = partial + (size_t)c * NO * n_obs_pad;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         if (idx[p] < n_obs_pad) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) out[(size_t)d * n_obs_pad + idx[p]] = acc[p][d];
+            for (int a = 0; a < NO; ++a) out[(size_t)a * n_obs_pad + idx[p]] = acc[p][a];
         }
     }
 }
@@ -286,8 +334,9 @@ __global__ void pack_obs_kernel(int64_t n, const double *__restrict__ pts,
     }
 }
 
+// rw words per record: [y n 0 ...]; padding points far away, zero strength.
 __global__ void pack_src_kernel(int64_t n, const double *__restrict__ pts,
-                                const double *__restrict__ ns, int64_t n_pad,
+                                const double *__restrict__ ns, int64_t n_pad, int rw,
                                 float *__restrict__ rec32, double *__restrict__ org,
                                 double *__restrict__ rec64) {
     const int64_t g = blockIdx.x;
@@ -304,21 +353,21 @@ __global__ void pack_src_kernel(int64_t n, const double *__restrict__ pts,
     for (int d = 0; d < 3; ++d) o[d] = wave_sum(p[d]) / cnt;
     if (lane == 0)
         for (int d = 0; d < 3; ++d) org[3 * g + d] = o[d];
-    float *r32 = rec32 + i * R;
-    double *r64 = rec64 + i * R;
+    float *r32 = rec32 + i * rw;
+    double *r64 = rec64 + i * rw;
     for (int d = 0; d < 3; ++d) {
         r32[d] = valid ? (float)(p[d] - o[d]) : (float)FAR_AWAY;
         r64[d] = valid ? p[d] : FAR_AWAY;
         r32[3 + d] = (float)nn[d];
         r64[3 + d] = nn[d];
     }
-    for (int d = 6; d < R; ++d) {
+    for (int d = 6; d < rw; ++d) {
         r32[d] = 0.0f;
         r64[d] = 0.0;
     }
 }
 
-// Strength scale and per-source extras for each kernel.
+// Strength scale and per-source extras for the point kernels.
 struct StrengthConsts {
     double scale;     // multiplies the raw strength
     double x0, x1;    // e0 = x0 (n.s'), e1 = x1 (n.s')
@@ -379,6 +428,60 @@ __global__ void gather_tris_kernel(int64_t n_pts, int64_t n_pad, int nq,
     write_strength<T>(rec + i * R, s[0], s[1], s[2], sc);
 }
 
+__device__ __forceinline__ int levi3(int i, int j, int k) { return (i - j) * (j - k) * (k - i) / 2; }
+
+// Surface-curl kernels: per source point the record fields that carry the
+// source triangle's surface curl S[d][I] = sum_b curl[b][I] v[9t+3b+d]
+// (kernels.cl:100-206) and the interpolated input.  tri_curl [n_tris][3][3]
+// (basis, component), indexed by DOF-space triangle.
+template <int KID, typename T>
+__global__ void gather_curl_kernel(int64_t n_pts, int64_t n_pad, int nq,
+                                   const int32_t *__restrict__ tri_of_pt,
+                                   const double *__restrict__ qphi, const double *__restrict__ wj,
+                                   const double *__restrict__ tri_curl,
+                                   const double *__restrict__ v, ElasticConsts c,
+                                   T *__restrict__ rec) {
+    constexpr int RW = rec_words<KID>();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pad) return;
+    T *r = rec + i * RW;
+    double S[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double in[3] = {0, 0, 0};
+    double f = 0.0;
+    if (i < n_pts) {
+        const int64_t t = tri_of_pt[i];
+        const int q = (int)(i % nq);
+        f = wj[i];
+        const double *vt = v + 9 * t;
+        const double *cu = tri_curl + 9 * t;
+        for (int d = 0; d < 3; ++d) {
+            in[d] = qphi[3 * q] * vt[d] + qphi[3 * q + 1] * vt[3 + d] + qphi[3 * q + 2] * vt[6 + d];
+            for (int I = 0; I < 3; ++I)
+                S[d][I] = cu[I] * vt[d] + cu[3 + I] * vt[3 + d] + cu[6 + I] * vt[6 + d];
+        }
+    }
+    if constexpr (KID == TCT_RT) {
+        for (int tt = 0; tt < 3; ++tt) {
+            double a = 0.0;
+            for (int d = 0; d < 3; ++d)
+                for (int I = 0; I < 3; ++I) a += levi3(I, tt, d) * S[d][I];
+            r[9 + tt] = (T)(c.CsRT1 * f * a);
+            r[6 + tt] = (T)(c.CsRT2 * f * in[tt]);
+        }
+    } else if constexpr (KID == TCT_RA) {
+        for (int d = 0; d < 3; ++d) r[6 + d] = (T)(f * in[d]);
+    } else {
+        const double g = f * c.CsRH0;
+        const double B = c.CsRH1 * (S[0][0] + S[1][1] + S[2][2]);
+        for (int d = 0; d < 3; ++d)
+            for (int mm = 0; mm < 3; ++mm) {
+                r[3 + d * 3 + mm] = (T)(g * S[d][mm]);
+                r[12 + d * 3 + mm] =
+                    (T)(g * ((d == mm ? B : 0.0) + c.CsRH2 * S[mm][d] - 2.0 * S[d][mm]));
+            }
+    }
+}
+
 __global__ void reduce_pts_kernel(int64_t n, int64_t n_obs_pad, int chunks,
                                   const double *__restrict__ partial, double *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -407,6 +510,47 @@ __global__ void scatter_tris_kernel(int64_t n_tris, int nq, const int32_t *__res
         double r = 0.0;
         for (int c = 0; c < chunks; ++c) r += partial[((size_t)c * 3 + d) * n_obs_pad + p];
         s += wj_obs[p] * qphi[3 * q + b] * r;
+    }
+    out[gid] = s;
+}
+
+// Galerkin scatter of the 9-moment kernels (RA, RH): applies the observation
+// triangle's surface curls.  tri_curl [n_tris][3][3] by DOF-space triangle.
+template <int KID>
+__global__ void scatter_curl_kernel(int64_t n_tris, int nq, const int32_t *__restrict__ pt_of_tri,
+                                    const double *__restrict__ qphi,
+                                    const double *__restrict__ wj_obs,
+                                    const double *__restrict__ tri_curl, int64_t n_obs_pad,
+                                    int chunks, const double *__restrict__ partial,
+                                    ElasticConsts c, double *__restrict__ out) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n_tris * 9) return;
+    const int64_t t = gid / 9;
+    const int bk = (int)(gid % 9);
+    const int b = bk / 3, k = bk % 3;
+    const double *cu = tri_curl + 9 * t + 3 * b;  // curl of basis b (3 components)
+    const int64_t p0 = (int64_t)pt_of_tri[t] * nq;
+    double s = 0.0;
+    for (int q = 0; q < nq; ++q) {
+        const int64_t p = p0 + q;
+        double M[9];
+        for (int a = 0; a < 9; ++a) {
+            double r = 0.0;
+            for (int ch = 0; ch < chunks; ++ch) r += partial[((size_t)ch * 9 + a) * n_obs_pad + p];
+            M[a] = r;
+        }
+        double val;
+        if constexpr (KID == TCT_RH) {
+            val = M[k * 3] * cu[0] + M[k * 3 + 1] * cu[1] + M[k * 3 + 2] * cu[2];
+        } else {
+            // sum_I curl[I] sum_s eps_{I s k} (CsRT1 v1_s + CsRT0 v2_s) - CsRT2 phi_b v3_k
+            double e = 0.0;
+            for (int I = 0; I < 3; ++I)
+                for (int ss = 0; ss < 3; ++ss)
+                    e += cu[I] * levi3(I, ss, k) * (c.CsRT1 * M[ss] + c.CsRT0 * M[3 + ss]);
+            val = e - c.CsRT2 * qphi[3 * q + b] * M[6 + k];
+        }
+        s += wj_obs[p] * val;
     }
     out[gid] = s;
 }
@@ -443,6 +587,8 @@ static KConst<T> kernel_consts(int kid, const double *params, double eps2) {
         k.c3 = (T)c.nu;
         k.c4 = (T)(3.0 * c.nu);
         break;
+    case TCT_RT: k.c0 = (T)(c.CsRT0 / c.CsRT1); break;
+    default: break;
     }
     return k;
 }
@@ -462,26 +608,33 @@ extern "C" int tct_far_pack_obs(int64_t n, const double *pts, const double *ns, 
     return TCT_OK;
 }
 
-extern "C" int tct_far_pack_src(int64_t n, const double *pts, const double *ns, int64_t n_pad,
-                                float *rec32, double *org, double *rec64, void *stream) {
+extern "C" int tct_far_pack_src_ex(int64_t n, const double *pts, const double *ns, int64_t n_pad,
+                                   int rec_words, float *rec32, double *org, double *rec64,
+                                   void *stream) {
     TCT_CHECK_ARG(n > 0 && n_pad >= n && n_pad % G == 0, "pack_src: bad sizes n=%lld n_pad=%lld",
                   (long long)n, (long long)n_pad);
+    TCT_CHECK_ARG(rec_words == 12 || rec_words == 24, "pack_src: rec_words %d", rec_words);
     TCT_CHECK_ARG(pts && ns && rec32 && org && rec64, "pack_src: null pointer");
     hipLaunchKernelGGL(pack_src_kernel, dim3((unsigned)(n_pad / G)), dim3(G), 0,
-                       as_stream(stream), n, pts, ns, n_pad, rec32, org, rec64);
+                       as_stream(stream), n, pts, ns, n_pad, rec_words, rec32, org, rec64);
     TCT_CHECK_LAUNCH("pack_src");
     return TCT_OK;
 }
 
-static int check_far_kernel(int kernel) {
+extern "C" int tct_far_pack_src(int64_t n, const double *pts, const double *ns, int64_t n_pad,
+                                float *rec32, double *org, double *rec64, void *stream) {
+    return tct_far_pack_src_ex(n, pts, ns, n_pad, R, rec32, org, rec64, stream);
+}
+
+static int check_point_kernel(int kernel) {
     if (kernel < TCT_U || kernel > TCT_H)
-        return set_error(TCT_EUNSUPPORTED, "farfield point kernel id %d not in U/T/A/H", kernel);
+        return set_error(TCT_EUNSUPPORTED, "point kernel id %d not in U/T/A/H", kernel);
     return TCT_OK;
 }
 
 extern "C" int tct_far_strength_pts(int kernel, int prec, int64_t n, const double *vec,
                                     const double *params_host, void *rec, void *stream) {
-    if (int e = check_far_kernel(kernel)) return e;
+    if (int e = check_point_kernel(kernel)) return e;
     TCT_CHECK_ARG(n >= 0 && vec && rec && params_host, "strength_pts: bad arguments");
     const int64_t n_pad = ceil_div(n, G) * G;
     StrengthConsts sc = strength_consts(kernel, params_host);
@@ -501,7 +654,7 @@ extern "C" int tct_far_gather_tris(int kernel, int prec, int64_t n_pts, int nq,
                                    const int32_t *tri_of_pt, const double *qphi, const double *wj,
                                    const double *v, const double *params_host, void *rec,
                                    void *stream) {
-    if (int e = check_far_kernel(kernel)) return e;
+    if (int e = check_point_kernel(kernel)) return e;
     TCT_CHECK_ARG(n_pts >= 0 && nq > 0 && n_pts % nq == 0, "gather_tris: n_pts %% nq != 0");
     TCT_CHECK_ARG(tri_of_pt && qphi && wj && v && rec && params_host, "gather_tris: null ptr");
     const int64_t n_pad = ceil_div(n_pts, G) * G;
@@ -518,12 +671,45 @@ extern "C" int tct_far_gather_tris(int kernel, int prec, int64_t n_pts, int nq,
     return TCT_OK;
 }
 
+extern "C" int tct_far_gather_curl(int kernel, int prec, int64_t n_pts, int nq,
+                                   const int32_t *tri_of_pt, const double *qphi, const double *wj,
+                                   const double *tri_curl, const double *v,
+                                   const double *params_host, void *rec, void *stream) {
+    TCT_CHECK_ARG(kernel >= TCT_RT && kernel <= TCT_RH, "gather_curl: kernel %d not RT/RA/RH",
+                  kernel);
+    TCT_CHECK_ARG(n_pts >= 0 && nq > 0 && n_pts % nq == 0, "gather_curl: n_pts %% nq != 0");
+    TCT_CHECK_ARG(tri_of_pt && qphi && wj && tri_curl && v && rec && params_host,
+                  "gather_curl: null pointer");
+    const int64_t n_pad = ceil_div(n_pts, G) * G;
+    const ElasticConsts c = elastic_consts(params_host);
+    const unsigned nb = (unsigned)ceil_div(n_pad, 256);
+    if (nb == 0) return TCT_OK;
+    hipStream_t st = as_stream(stream);
+#define TCT_GC(KID)                                                                            \
+    if (prec == TCT_F32)                                                                       \
+        hipLaunchKernelGGL((gather_curl_kernel<KID, float>), dim3(nb), dim3(256), 0, st, n_pts, \
+                           n_pad, nq, tri_of_pt, qphi, wj, tri_curl, v, c, (float *)rec);      \
+    else                                                                                       \
+        hipLaunchKernelGGL((gather_curl_kernel<KID, double>), dim3(nb), dim3(256), 0, st,      \
+                           n_pts, n_pad, nq, tri_of_pt, qphi, wj, tri_curl, v, c, (double *)rec)
+    if (kernel == TCT_RT) {
+        TCT_GC(TCT_RT);
+    } else if (kernel == TCT_RA) {
+        TCT_GC(TCT_RA);
+    } else {
+        TCT_GC(TCT_RH);
+    }
+#undef TCT_GC
+    TCT_CHECK_LAUNCH("gather_curl");
+    return TCT_OK;
+}
+
 extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float *loc32,
                            const double *obs_org, const double *obs_soa64, int64_t n_src_pad,
                            const void *rec, const double *src_org, const double *params_host,
                            double eps2, const tct_tile_cfg *cfg, double *out_partial,
                            void *stream) {
-    if (int e = check_far_kernel(kernel)) return e;
+    TCT_CHECK_ARG(kernel >= TCT_U && kernel <= TCT_RH, "far_p2p: kernel id %d", kernel);
     TCT_CHECK_ARG(eps2 >= 0.0, "far_p2p: eps2 must be >= 0");
     TCT_CHECK_ARG(n_obs_pad > 0 && n_obs_pad % G == 0 && n_src_pad > 0 && n_src_pad % G == 0,
                   "far_p2p: sizes must be positive multiples of %d", G);
@@ -560,7 +746,10 @@ extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float 
     case TCT_U: TCT_P_SWITCH(MAC, TCT_U); break;        \
     case TCT_T: TCT_P_SWITCH(MAC, TCT_T); break;        \
     case TCT_A: TCT_P_SWITCH(MAC, TCT_A); break;        \
-    default: TCT_P_SWITCH(MAC, TCT_H); break;           \
+    case TCT_H: TCT_P_SWITCH(MAC, TCT_H); break;        \
+    case TCT_RT: TCT_P_SWITCH(MAC, TCT_RT); break;      \
+    case TCT_RA: TCT_P_SWITCH(MAC, TCT_RA); break;      \
+    default: TCT_P_SWITCH(MAC, TCT_RH); break;          \
     }
     if (prec == TCT_F32) {
         TCT_K_SWITCH(TCT_F32_CASE)
@@ -598,5 +787,31 @@ extern "C" int tct_far_scatter_tris(int64_t n_tris, int nq, const int32_t *pt_of
                        0, as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, n_obs_pad,
                        chunks, partial, out);
     TCT_CHECK_LAUNCH("scatter_tris");
+    return TCT_OK;
+}
+
+extern "C" int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const int32_t *pt_of_tri,
+                                    const double *qphi, const double *wj_obs,
+                                    const double *tri_curl, int64_t n_obs_pad, int chunks,
+                                    const double *partial, const double *params_host,
+                                    double *out, void *stream) {
+    TCT_CHECK_ARG(kernel == TCT_RA || kernel == TCT_RH, "scatter_curl: kernel %d not RA/RH",
+                  kernel);
+    TCT_CHECK_ARG(n_tris >= 0 && nq > 0 && chunks >= 1 && n_tris * nq <= n_obs_pad,
+                  "scatter_curl: bad sizes");
+    TCT_CHECK_ARG(pt_of_tri && qphi && wj_obs && tri_curl && partial && out && params_host,
+                  "scatter_curl: null pointer");
+    if (n_tris == 0) return TCT_OK;
+    const ElasticConsts c = elastic_consts(params_host);
+    const unsigned nb = (unsigned)ceil_div(n_tris * 9, 256);
+    if (kernel == TCT_RH)
+        hipLaunchKernelGGL(scatter_curl_kernel<TCT_RH>, dim3(nb), dim3(256), 0,
+                           as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl,
+                           n_obs_pad, chunks, partial, c, out);
+    else
+        hipLaunchKernelGGL(scatter_curl_kernel<TCT_RA>, dim3(nb), dim3(256), 0,
+                           as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl,
+                           n_obs_pad, chunks, partial, c, out);
+    TCT_CHECK_LAUNCH("scatter_curl");
     return TCT_OK;
 }

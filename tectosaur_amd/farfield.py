@@ -89,9 +89,10 @@ class FarfieldEngine:
     def __init__(self, K_name, params, obs_pts, obs_ns, src_pts, src_ns,
                  precisions=('f32', 'f64'), device=None, tile_cfg=None):
         K = KERNELS[K_name]
-        if K.surf_curl_obs or K.surf_curl_src:
-            raise N.TctError('%s has no point-to-point form' % K_name)
         self.K = K
+        # record words per source point and sums per observation point
+        self.rec_words = 24 if K_name == 'elasticRH3' else REC
+        self.n_out = 9 if K_name in ('elasticRA3', 'elasticRH3') else 3
         self.kid = K.kid
         self.params = N.params_array(params)
         self.device = device if device is not None else N.require_device()
@@ -114,11 +115,11 @@ class FarfieldEngine:
         self.obs_soa64 = torch.empty(6 * self.n_obs_pad, **f64)
         N.call('tct_far_pack_obs', self.n_obs, N.ptr(op), N.ptr(on), self.n_obs_pad,
                N.ptr(self.loc32), N.ptr(self.obs_org), N.ptr(self.obs_soa64), st)
-        self.rec32 = torch.empty(REC * self.n_src_pad, dtype=torch.float32, device=dev)
+        self.rec32 = torch.empty(self.rec_words * self.n_src_pad, dtype=torch.float32, device=dev)
         self.src_org = torch.empty(3 * (self.n_src_pad // GROUP), **f64)
-        self.rec64 = torch.empty(REC * self.n_src_pad, **f64)
-        N.call('tct_far_pack_src', self.n_src, N.ptr(sp), N.ptr(sn), self.n_src_pad,
-               N.ptr(self.rec32), N.ptr(self.src_org), N.ptr(self.rec64), st)
+        self.rec64 = torch.empty(self.rec_words * self.n_src_pad, **f64)
+        N.call('tct_far_pack_src_ex', self.n_src, N.ptr(sp), N.ptr(sn), self.n_src_pad,
+               self.rec_words, N.ptr(self.rec32), N.ptr(self.src_org), N.ptr(self.rec64), st)
         # rounding bound of one point stored against two different origins
         self.eps2 = (8.0 * U32 * (group_extent(obs_pts) + group_extent(src_pts))) ** 2
         self.tile_cfg = {}
@@ -143,12 +144,14 @@ class FarfieldEngine:
         cfg = self.tile_cfg[prec]
         key = (prec, cfg.src_chunks)
         if key not in self._partial:
-            self._partial[key] = torch.empty(cfg.src_chunks * 3 * self.n_obs_pad,
+            self._partial[key] = torch.empty(cfg.src_chunks * self.n_out * self.n_obs_pad,
                                              dtype=torch.float64, device=self.device)
         return self._partial[key]
 
     def set_strengths(self, vec_dev, prec):
         """vec_dev: float64 device tensor [n_src, 3] in packing order."""
+        if self.K.surf_curl_obs or self.K.surf_curl_src:
+            raise N.TctError('%s has no point-to-point form' % self.K.name)
         N.call('tct_far_strength_pts', self.kid, self._prec_id(prec), self.n_src,
                N.ptr(vec_dev), self.params.ctypes.data, N.ptr(self.rec(prec)), N.stream_ptr())
 
@@ -156,6 +159,18 @@ class FarfieldEngine:
         N.call('tct_far_gather_tris', self.kid, self._prec_id(prec), self.n_src, nq,
                N.ptr(tri_of_pt), N.ptr(qphi), N.ptr(wj), N.ptr(v_dev), self.params.ctypes.data,
                N.ptr(self.rec(prec)), N.stream_ptr())
+
+    def gather_curl(self, v_dev, tri_of_pt, qphi, wj, tri_curl, nq, prec):
+        N.call('tct_far_gather_curl', self.kid, self._prec_id(prec), self.n_src, nq,
+               N.ptr(tri_of_pt), N.ptr(qphi), N.ptr(wj), N.ptr(tri_curl), N.ptr(v_dev),
+               self.params.ctypes.data, N.ptr(self.rec(prec)), N.stream_ptr())
+
+    def scatter_curl(self, prec, n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl, out_dev):
+        cfg = self.tile_cfg[prec]
+        N.call('tct_far_scatter_curl', self.kid, n_tris, nq, N.ptr(pt_of_tri), N.ptr(qphi),
+               N.ptr(wj_obs), N.ptr(tri_curl), self.n_obs_pad, cfg.src_chunks,
+               N.ptr(self.partial(prec)), self.params.ctypes.data, N.ptr(out_dev),
+               N.stream_ptr())
 
     def run(self, prec, cfg=None):
         """Launch the N-body kernel; returns the partial-sum buffer."""

@@ -14,8 +14,12 @@ sum is computed as three device steps per matvec:
 
 For the point kernels U, T, A, H this is the same sum in a different order
 (the kernel of a pair depends only on the two points and normals).  The
-surface-curl kernels RT, RA, RH have no point form and are not supported
-by this operator.
+surface-curl kernels RT, RA, RH (the vector codes of
+tectosaur/kernels/kernels.cl:100-206) depend on whole triangles through the
+basis surface curls; they are rewritten as point pairs with richer source
+records (the source triangle's curl contracted with the input, folded in by
+the gather) and, for RA and RH, 9 moments per observation point that the
+scatter contracts with the observation triangle's curls.
 """
 import numpy as np
 import torch
@@ -30,8 +34,7 @@ class TriToTriDirectFarfieldOp:
     def __init__(self, nq_far, K_name, params, pts, tris, float_type, obs_subset, src_subset,
                  tile_cfg=None, precisions=None):
         K = KERNELS[K_name]
-        if K.surf_curl_obs or K.surf_curl_src:
-            raise NotImplementedError('%s: surface-curl farfield is not supported' % K_name)
+        self.curl = K.surf_curl_obs or K.surf_curl_src
         pts = np.asarray(pts, dtype=np.float64)
         tris = np.asarray(tris)
         obs_subset = np.asarray(obs_subset, dtype=np.int64)
@@ -64,6 +67,16 @@ class TriToTriDirectFarfieldOp:
             wj = (jac[:, None] * qw[None, :]).reshape(-1)
             return order, qp, qn, wj
 
+        def curls(subset):
+            # surface curls of the basis functions, DOF-space order
+            # (integral_primitives.cl:95-112): curl_b = (g_b0 e2 - g_b1 e1) / J
+            v = pts[tris[subset]]
+            e1 = v[:, 1] - v[:, 0]
+            e2 = v[:, 2] - v[:, 0]
+            J = np.linalg.norm(np.cross(v[:, 2] - v[:, 0], v[:, 2] - v[:, 1]), axis=1)[:, None]
+            c = np.stack([(-e2 + e1) / J, e2 / J, -e1 / J], axis=1)
+            return np.ascontiguousarray(c.reshape((-1, 9)))
+
         o_order, o_pts, o_ns, o_wj = side(obs_subset)
         s_order, s_pts, s_ns, s_wj = side(src_subset)
         if precisions is None:
@@ -80,6 +93,9 @@ class TriToTriDirectFarfieldOp:
         self.qphi = torch.as_tensor(np.ascontiguousarray(phi), **f64)
         self.wj_src = torch.as_tensor(s_wj, **f64)
         self.wj_obs = torch.as_tensor(o_wj, **f64)
+        if self.curl:
+            self.curl_src = torch.as_tensor(curls(src_subset), **f64)
+            self.curl_obs = torch.as_tensor(curls(obs_subset), **f64)
 
     @property
     def pair_evals(self):
@@ -91,10 +107,18 @@ class TriToTriDirectFarfieldOp:
         if out_dev is None:
             out_dev = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
         e = self.engine
-        e.gather_tris(v_dev, self.tri_of_pt, self.qphi, self.wj_src, self.nq, prec)
+        if self.curl:
+            e.gather_curl(v_dev, self.tri_of_pt, self.qphi, self.wj_src, self.curl_src, self.nq,
+                          prec)
+        else:
+            e.gather_tris(v_dev, self.tri_of_pt, self.qphi, self.wj_src, self.nq, prec)
         e.run(prec)
-        e.scatter_tris(prec, self.n_obs, self.nq, self.pt_of_tri, self.qphi, self.wj_obs,
-                       out_dev)
+        if e.n_out == 9:
+            e.scatter_curl(prec, self.n_obs, self.nq, self.pt_of_tri, self.qphi, self.wj_obs,
+                           self.curl_obs, out_dev)
+        else:
+            e.scatter_tris(prec, self.n_obs, self.nq, self.pt_of_tri, self.qphi, self.wj_obs,
+                           out_dev)
         return out_dev
 
     def dot(self, v):

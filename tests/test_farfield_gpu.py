@@ -173,9 +173,44 @@ def test_deterministic():
     assert np.array_equal(a, b)
 
 
-def test_unsupported_kernel_raises():
+CURLK = ['elasticRT3', 'elasticRA3', 'elasticRH3']
+
+
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+@pytest.mark.parametrize('K', CURLK)
+@pytest.mark.parametrize('nq', [1, 2])
+def test_surface_curl_tri_tri_separated(oracle, K, prec, nq):
+    # matrix_free.cl farfield_tris_to_tris{K} with the vector codes of kernels.cl:100-206
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
-    pts, tris = cfg1_mesh()
-    with pytest.raises(NotImplementedError):
-        TriToTriDirectFarfieldOp(1, 'elasticRH3', [3e10, 0.25], pts, tris, np.float32,
-                                 np.arange(4), np.arange(4))
+    (pts, tris), s1, s2 = two_surfaces()
+    ft = np.float32 if prec == 'f32' else np.float64
+    op = TriToTriDirectFarfieldOp(nq, K, [1.0, 0.25], pts, tris, ft, s1, s2)
+    v = np.random.default_rng(21).random(op.shape[1])
+    out = op.dot(v)
+    qx, qw = gauss2d_tri(nq)
+    ref = oracle.farfield_tris(K, qx, qw, pts, tris[s1], tris[s2], v, [1.0, 0.25])
+    assert rel_l2(out, ref) <= TOL[prec]
+
+
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+@pytest.mark.parametrize('K', CURLK)
+def test_surface_curl_tri_tri_self(oracle, K, prec):
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = mesh.make_rect(10, 8, [[-1, 0, 1], [-1, 0, -1], [1, 0, -1], [1, 0, 1]])
+    pts = pts + 0.05 * np.random.default_rng(22).normal(size=pts.shape)
+    n = tris.shape[0]
+    ft = np.float32 if prec == 'f32' else np.float64
+    op = TriToTriDirectFarfieldOp(2, K, [3e10, 0.25], pts, tris, ft, np.arange(n), np.arange(n))
+    v = np.random.default_rng(23).normal(size=op.shape[1])
+    out = op.dot(v)
+    qx, qw = gauss2d_tri(2)
+    ref = oracle.farfield_tris(K, qx, qw, pts, tris, tris, v, [3e10, 0.25])
+    assert rel_l2(out, ref) <= TOL[prec]
+
+
+def test_point_form_refuses_surface_curl_kernels():
+    from tectosaur_amd.farfield import farfield_pts_direct
+    from tectosaur_amd import _native as N
+    p = np.random.default_rng(0).random((10, 3))
+    with pytest.raises(N.TctError):
+        farfield_pts_direct('elasticRH3', p, p, p, p, p.reshape(-1), [1.0, 0.25], np.float32)

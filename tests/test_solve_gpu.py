@@ -15,7 +15,7 @@ ORD = dict(nq_coincident=4, nq_edge_adj=4, nq_vert_adjacent=3, nq_far=1, nq_near
            near_threshold=2.5, quad_mass_order=3)
 
 
-def oracle_apply(oracle, pts, tris, u, nu=0.25):
+def oracle_apply(oracle, pts, tris, u, nu=0.25, K_far='elasticRH3'):
     o = ORD
     rules = {'co': triangle_rules.coincident_quad(o['nq_coincident']),
              'ea': triangle_rules.edge_adj_quad(o['nq_edge_adj']),
@@ -24,13 +24,13 @@ def oracle_apply(oracle, pts, tris, u, nu=0.25):
              'far': gauss4d_tri(o['nq_far'], o['nq_far'])}
     blocks = oracle.nearfield_blocks(pts, tris, o['nq_coincident'], o['nq_edge_adj'],
                                      o['nq_vert_adjacent'], o['nq_far'], o['nq_near'],
-                                     o['near_threshold'], 'elasticRH3', 'elasticH3', [1.0, nu],
+                                     o['near_threshold'], 'elasticRH3', K_far, [1.0, nu],
                                      rules)
     y = np.zeros(tris.shape[0] * 9)
     for prs, corr, _ in blocks.values():
         y += oracle.bcoo_matvec(prs[:, 0], prs[:, 1], corr, u, (y.shape[0], y.shape[0]))
     qx, qw = gauss2d_tri(o['nq_far'])
-    y += oracle.farfield_tris('elasticH3', qx, qw, pts, tris, tris, u, [1.0, nu])
+    y += oracle.farfield_tris(K_far, qx, qw, pts, tris, tris, u, [1.0, nu])
     return y
 
 
@@ -40,16 +40,17 @@ def small_problem(n_tris=200, seed=5):
     return pts, tris, request_traction(req, tris.shape[0])
 
 
+@pytest.mark.parametrize('K_far', ['elasticRH3', 'elasticH3'])
 @pytest.mark.parametrize('mode', ['f64', 'mixed'])
-def test_solve_residual_under_oracle_operator(oracle, mode):
+def test_solve_residual_under_oracle_operator(oracle, mode, K_far):
     from tectosaur_amd.solver import solve_constrained
     pts, tris, t = small_problem()
-    prob = TractionSlipProblem(pts, tris, t, orders=ORD)
+    prob = TractionSlipProblem(pts, tris, t, orders=ORD, K_far=K_far)
     u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=1e-8, mode=mode, restart=60)
     assert res.converged and res.rel_residual <= 1e-8
     u = u.cpu().numpy()
     b = prob.cm.T @ prob.rhs.cpu().numpy()
-    r = b - prob.cm.T @ oracle_apply(oracle, pts, tris, u)
+    r = b - prob.cm.T @ oracle_apply(oracle, pts, tris, u, K_far=K_far)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.1e-8
     if mode == 'mixed':
         assert res.n_matvec['f32'] > res.n_matvec['f64']
