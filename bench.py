@@ -90,7 +90,7 @@ def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
         {'f64_vs_oracle_rel_l2': rl2(out64_rows), 'f32_vs_oracle_rel_l2': rl2(out32_rows)}
 
 
-def serve_bench(n_req, max_tris, rate, seed=0, max_active=32):
+def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512):
     """Served traction -> slip solves: one Poisson stream of fault requests
     through the continuous-batching mixed-precision server and through the
     plain FP64 one-at-a-time path; latency = finish - arrival (queueing,
@@ -100,13 +100,14 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32):
     from tectosaur_amd.traction_slip import random_requests
     warm = random_requests(2, rate=0.0, seed=seed + 99, n_min=512, n_max=1024)
     Server(mode='mixed', max_active=2).run(warm, realtime=False)
-    Server(mode='f64', max_active=1).run(warm, realtime=False)
-    reqs = random_requests(n_req, rate=rate, seed=seed, n_min=512, n_max=max_tris)
+    Server(mode='f64', max_active=1, prep_workers=0).run(warm, realtime=False)
+    reqs = random_requests(n_req, rate=rate, seed=seed, n_min=min_tris, n_max=max_tris)
     out = {'workload': '%d fault requests, Poisson %.2f/s, strike U(0,360) dip U(20,90) '
-                       'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near /'
-                       ' H3 far, tol 1e-8' % (n_req, rate, 512, max_tris)}
-    for name, mode, ma in (('batched_mixed', 'mixed', max_active), ('sequential_f64', 'f64', 1)):
-        srv = Server(mode=mode, max_active=ma)
+                       'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near and'
+                       ' far, tol 1e-8' % (n_req, rate, min_tris, max_tris)}
+    for name, mode, ma, pw in (('batched_mixed', 'mixed', max_active, 4),
+                               ('sequential_f64', 'f64', 1, 0)):
+        srv = Server(mode=mode, max_active=ma, prep_workers=pw)
         torch.cuda.synchronize()
         recs = srv.run(reqs, realtime=True)
         sm = summarize(recs, srv.makespan)

@@ -210,6 +210,16 @@ int tct_find_near_pairs(int64_t n_a, const double *ca, const double *ra, int64_t
                         const double *cb, const double *rb, double threshold, int64_t *out,
                         int64_t cap, int64_t *n_out);
 
+/* tct_split_adjacent_close: classify near pairs by shared vertices
+ * (tectosaur/mesh/fast_find_nearfield.cpp:178-217).  pairs [n][2] index
+ * tris_a / tris_b ([*][3] int64).  Writes, in input order, near [*][2]
+ * (no shared vertex), va [*][4] (i, j, corner_i, corner_j) and ea [*][6]
+ * (i, j, two corner pairs); coincident pairs are dropped.  Each output
+ * needs room for n_pairs rows; counts[3] receives the three sizes. */
+int tct_split_adjacent_close(int64_t n_pairs, const int64_t *pairs, const int64_t *tris_a,
+                             const int64_t *tris_b, int64_t *near, int64_t *va, int64_t *ea,
+                             int64_t *counts);
+
 /* ---------------- host-side constraint matrix (CPU) ---------------------
  * Reference: build_constraint_matrix / reduce_constraints
  * (tectosaur/fast_constraints.cpp:120-300, tectosaur/constraints.py:10-17).

@@ -59,6 +59,7 @@ SIGNATURES = {
     'tct_bsr9_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_csr_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_find_near_pairs': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _vp, _i64, _vp],
+    'tct_split_adjacent_close': [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'tct_build_constraint_matrix': [_i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
                                     _vp, _vp, _vp, _vp, _vp],
 }

@@ -135,14 +135,18 @@ def continuity_constraints(pts, tris, fault_start_idx, tensor_dim=3):
 
 
 def find_free_edges(tris):
-    """(triangle, local edge) of every edge used by exactly one triangle."""
-    edges = collections.OrderedDict()
-    for i, t in enumerate(tris):
-        for d in range(3):
-            a, b = int(t[d]), int(t[(d + 1) % 3])
-            key = (min(a, b), max(a, b))
-            edges.setdefault(key, []).append((i, d))
-    return [e[0] for e in edges.values() if len(e) == 1]
+    """(triangle, local edge) of every edge used by exactly one triangle, in
+    the order the reference's dict walk yields them (first appearance)."""
+    t = np.asarray(tris, dtype=np.int64)
+    if t.shape[0] == 0:
+        return []
+    a = t.reshape(-1)
+    b = t[:, [1, 2, 0]].reshape(-1)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    key = lo * (int(t.max()) + 1) + hi
+    _, first, cnt = np.unique(key, return_index=True, return_counts=True)
+    idx = np.sort(first[cnt == 1])
+    return [(int(i // 3), int(i % 3)) for i in idx]
 
 
 def free_edge_dofs(tris, free_edges):

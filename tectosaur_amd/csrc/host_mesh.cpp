@@ -140,3 +140,67 @@ extern "C" int tct_find_near_pairs(int64_t n_a, const double *ca, const double *
         }
     return TCT_OK;
 }
+
+// Reference: split_adjacent_close (tectosaur/mesh/fast_find_nearfield.cpp
+// :178-217).  For each pair count the shared vertices (a pair with three
+// shared vertices is coincident and dropped); shared corners are recorded
+// in (corner_a, corner_b) row-major order.  Output order follows the input.
+// near [cap][2], va [cap][4], ea [cap][6] (cap >= n_pairs suffices);
+// counts[3] receives the class sizes.
+extern "C" int tct_split_adjacent_close(int64_t n_pairs, const int64_t *pairs,
+                                        const int64_t *tris_a, const int64_t *tris_b,
+                                        int64_t *near, int64_t *va, int64_t *ea,
+                                        int64_t *counts) {
+    if (n_pairs < 0 || !counts || (n_pairs > 0 && (!pairs || !tris_a || !tris_b || !near ||
+                                                    !va || !ea)))
+        return tct::set_error(TCT_EARG, "split_adjacent_close: bad arguments");
+    std::vector<int8_t> cls(n_pairs);
+    std::vector<int8_t> corner(4 * n_pairs);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n_pairs; k++) {
+        const int64_t *a = tris_a + 3 * pairs[2 * k];
+        const int64_t *b = tris_b + 3 * pairs[2 * k + 1];
+        int found = 0;
+        for (int d1 = 0; d1 < 3; d1++)
+            for (int d2 = 0; d2 < 3; d2++) {
+                if (a[d1] != b[d2]) continue;
+                if (found < 2) {
+                    corner[4 * k + 2 * found] = (int8_t)d1;
+                    corner[4 * k + 2 * found + 1] = (int8_t)d2;
+                }
+                found++;
+            }
+        cls[k] = (int8_t)(found >= 3 ? 3 : found);
+    }
+    int64_t nn = 0, nv = 0, ne = 0;
+    for (int64_t k = 0; k < n_pairs; k++) {
+        const int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+        const int8_t *c = corner.data() + 4 * k;
+        switch (cls[k]) {
+        case 0:
+            near[2 * nn] = i;
+            near[2 * nn + 1] = j;
+            nn++;
+            break;
+        case 1:
+            va[4 * nv] = i;
+            va[4 * nv + 1] = j;
+            va[4 * nv + 2] = c[0];
+            va[4 * nv + 3] = c[1];
+            nv++;
+            break;
+        case 2:
+            ea[6 * ne] = i;
+            ea[6 * ne + 1] = j;
+            for (int q = 0; q < 4; q++) ea[6 * ne + 2 + q] = c[q];
+            ne++;
+            break;
+        default:
+            break;
+        }
+    }
+    counts[0] = nn;
+    counts[1] = nv;
+    counts[2] = ne;
+    return TCT_OK;
+}

@@ -50,14 +50,17 @@ def round_up(n, m):
 
 def group_extent(p):
     """Largest distance of a point from the mean of its 64-point group."""
+    p = np.asarray(p, dtype=np.float64)
     n = p.shape[0]
-    npad = round_up(n, GROUP)
-    ext = 0.0
-    for g0 in range(0, npad, GROUP):
-        blk = p[g0:min(g0 + GROUP, n)]
-        if blk.shape[0]:
-            ext = max(ext, float(np.max(np.linalg.norm(blk - blk.mean(axis=0), axis=1))))
-    return ext
+    if n == 0:
+        return 0.0
+    q = np.full((round_up(n, GROUP), 3), np.nan)
+    q[:n] = p
+    q = q.reshape((-1, GROUP, 3))
+    cnt = np.sum(~np.isnan(q[:, :, 0]), axis=1)
+    mean = np.nansum(q, axis=1) / cnt[:, None]
+    d = np.linalg.norm(q - mean[:, None, :], axis=2)
+    return float(np.nanmax(d))
 
 
 def default_tile_cfg(n_obs_pad, n_src_pad, prec, device=None):
@@ -87,7 +90,7 @@ class FarfieldEngine:
     """
 
     def __init__(self, K_name, params, obs_pts, obs_ns, src_pts, src_ns,
-                 precisions=('f32', 'f64'), device=None, tile_cfg=None):
+                 precisions=('f32', 'f64'), device=None, tile_cfg=None, extents=None):
         K = KERNELS[K_name]
         self.K = K
         # record words per source point and sums per observation point
@@ -121,7 +124,9 @@ class FarfieldEngine:
         N.call('tct_far_pack_src_ex', self.n_src, N.ptr(sp), N.ptr(sn), self.n_src_pad,
                self.rec_words, N.ptr(self.rec32), N.ptr(self.src_org), N.ptr(self.rec64), st)
         # rounding bound of one point stored against two different origins
-        self.eps2 = (8.0 * U32 * (group_extent(obs_pts) + group_extent(src_pts))) ** 2
+        if extents is None:
+            extents = (group_extent(obs_pts), group_extent(src_pts))
+        self.eps2 = (8.0 * U32 * (extents[0] + extents[1])) ** 2
         self.tile_cfg = {}
         for prec in ('f32', 'f64'):
             cfg = tile_cfg.get(prec) if isinstance(tile_cfg, dict) else tile_cfg

@@ -45,22 +45,21 @@ def split_adjacent_close(close_pairs, tris_A, tris_B):
     """(near [n,2], vertex-adjacent [n,4] (i, j, corner_i, corner_j),
     edge-adjacent [n,6] (i, j, corner pair 1, corner pair 2)); pairs sharing
     all three vertices (coincident) are dropped.  Shared corners are listed
-    in (corner_i, corner_j) row-major order, as the reference's loop does."""
-    cp = np.asarray(close_pairs, dtype=np.int64).reshape((-1, 2))
-    if cp.shape[0] == 0:
+    in (corner_i, corner_j) row-major order, as the reference's loop does.
+    Runs in C++ (host_mesh.cpp) through the C ABI."""
+    cp = np.ascontiguousarray(np.asarray(close_pairs, dtype=np.int64).reshape((-1, 2)))
+    n = cp.shape[0]
+    if n == 0:
         return (np.zeros((0, 2), np.int64), np.zeros((0, 4), np.int64),
                 np.zeros((0, 6), np.int64))
-    ta = np.asarray(tris_A)[cp[:, 0]]
-    tb = np.asarray(tris_B)[cp[:, 1]]
-    eq = (ta[:, :, None] == tb[:, None, :]).reshape((-1, 9))
-    cnt = eq.sum(axis=1)
-    order = np.argsort(~eq, axis=1, kind='stable')  # matched slots first, row-major
-    d1 = order // 3
-    d2 = order % 3
-    near = cp[cnt == 0]
-    vsel = cnt == 1
-    va = np.concatenate([cp[vsel], d1[vsel, :1], d2[vsel, :1]], axis=1)
-    esel = cnt == 2
-    ea = np.concatenate([cp[esel], d1[esel, :1], d2[esel, :1], d1[esel, 1:2], d2[esel, 1:2]],
-                        axis=1)
-    return near, va, ea
+    ta = np.ascontiguousarray(np.asarray(tris_A, dtype=np.int64))
+    tb = np.ascontiguousarray(np.asarray(tris_B, dtype=np.int64))
+    if cp[:, 0].max() >= ta.shape[0] or cp[:, 1].max() >= tb.shape[0] or cp.min() < 0:
+        raise IndexError('pair index out of range')
+    near = np.empty((n, 2), np.int64)
+    va = np.empty((n, 4), np.int64)
+    ea = np.empty((n, 6), np.int64)
+    counts = np.zeros(3, np.int64)
+    N.call('tct_split_adjacent_close', n, cp.ctypes.data, ta.ctypes.data, tb.ctypes.data,
+           near.ctypes.data, va.ctypes.data, ea.ctypes.data, counts.ctypes.data)
+    return near[:counts[0]].copy(), va[:counts[1]].copy(), ea[:counts[2]].copy()

@@ -68,10 +68,33 @@ def to_dof_space(tri_indices, obs_subset, src_subset):
     return out
 
 
+def nearfield_plan(pts, tris, obs_subset, src_subset, near_threshold):
+    """Host-side pair classification (no device work): coincident,
+    edge-adjacent (rotated), vertex-adjacent and near pairs, in triangle and
+    DOF space (nearfield_op.py:95-114)."""
+    pts = np.asarray(pts, dtype=np.float64)
+    tris = np.asarray(tris, dtype=np.int64)
+    obs_subset = np.asarray(obs_subset, dtype=np.int64)
+    src_subset = np.asarray(src_subset, dtype=np.int64)
+    co_tris = np.intersect1d(obs_subset, src_subset)
+    co_indices = np.stack([co_tris, co_tris], axis=1)
+    co_dofs = to_dof_space(co_indices, obs_subset, src_subset)
+    close = find_near_adj.find_close_or_touching(pts, tris[obs_subset], pts, tris[src_subset],
+                                                 near_threshold)
+    near_dofs, va_dofs, ea_dofs = find_near_adj.split_adjacent_close(
+        close, tris[obs_subset], tris[src_subset])
+    near = to_tri_space(near_dofs, obs_subset, src_subset)
+    va = to_tri_space(va_dofs, obs_subset, src_subset)
+    va = np.concatenate([va, np.zeros((va.shape[0], 1), dtype=np.int64)], axis=1)
+    ea = resolve_ea_rotation(tris, to_tri_space(ea_dofs, obs_subset, src_subset))
+    return dict(co=(co_indices, co_dofs), ea=(ea, ea_dofs[:, :2]), va=(va, va_dofs[:, :2]),
+                near=(near, near_dofs))
+
+
 class RegularizedNearfieldIntegralOp:
     def __init__(self, pts, tris, obs_subset, src_subset, nq_coincident, nq_edge_adj,
                  nq_vert_adjacent, nq_far, nq_near, near_threshold, K_near_name, K_far_name,
-                 params, float_type, keep_classes=False):
+                 params, float_type, keep_classes=False, plan=None, keep_uncorrected=True):
         pts = np.asarray(pts, dtype=np.float64)
         tris = np.asarray(tris, dtype=np.int64)
         obs_subset = np.asarray(obs_subset, dtype=np.int64)
@@ -81,19 +104,8 @@ class RegularizedNearfieldIntegralOp:
         dev = self.device
         pairs_int = PairsIntegrator(K_near_name, params, float_type, nq_far, nq_near, pts, tris)
         corr_int = PairsIntegrator(K_far_name, params, float_type, nq_far, nq_near, pts, tris)
-
-        co_tris = np.intersect1d(obs_subset, src_subset)
-        co_indices = np.stack([co_tris, co_tris], axis=1)
-        co_dofs = to_dof_space(co_indices, obs_subset, src_subset)
-
-        close = find_near_adj.find_close_or_touching(pts, tris[obs_subset], pts,
-                                                     tris[src_subset], near_threshold)
-        near_dofs, va_dofs, ea_dofs = find_near_adj.split_adjacent_close(
-            close, tris[obs_subset], tris[src_subset])
-        near = to_tri_space(near_dofs, obs_subset, src_subset)
-        va = to_tri_space(va_dofs, obs_subset, src_subset)
-        va = np.concatenate([va, np.zeros((va.shape[0], 1), dtype=np.int64)], axis=1)
-        ea = resolve_ea_rotation(tris, to_tri_space(ea_dofs, obs_subset, src_subset))
+        if plan is None:
+            plan = nearfield_plan(pts, tris, obs_subset, src_subset, near_threshold)
 
         from . import triangle_rules
         from .quadrature import gauss4d_tri
@@ -109,10 +121,10 @@ class RegularizedNearfieldIntegralOp:
             'near': pairs_int.near_q,
         }
         far_q = corr_int.far_q
-        classes = (('co', co_indices, co_dofs, True, True),
-                   ('ea', ea, ea_dofs[:, :2], True, False),
-                   ('va', va, va_dofs[:, :2], True, False),
-                   ('near', near, near_dofs, False, False))
+        classes = (('co',) + plan['co'] + (True, True),
+                   ('ea',) + plan['ea'] + (True, False),
+                   ('va',) + plan['va'] + (True, False),
+                   ('near',) + plan['near'] + (False, False))
         rows, cols, corrected, uncorrected = [], [], [], []
         self.counts = {}
         self.classes = {}
@@ -131,7 +143,8 @@ class RegularizedNearfieldIntegralOp:
             rows.append(torch.as_tensor(dofs[:, 0], device=dev))
             cols.append(torch.as_tensor(dofs[:, 1], device=dev))
             corrected.append(diff)
-            uncorrected.append(acc)
+            if keep_uncorrected:
+                uncorrected.append(acc)
             if keep_classes:
                 self.classes[name] = (prs, dofs, acc, corr)
             del corr
@@ -139,7 +152,8 @@ class RegularizedNearfieldIntegralOp:
             r = torch.cat(rows)
             c = torch.cat(cols)
             self.mat = bcoo_to_bsr_dev(r, c, torch.cat(corrected), self.shape)
-            self.mat_no_correction = bcoo_to_bsr_dev(r, c, torch.cat(uncorrected), self.shape)
+            self.mat_no_correction = (bcoo_to_bsr_dev(r, c, torch.cat(uncorrected), self.shape)
+                                      if keep_uncorrected else None)
         else:
             z = torch.zeros((0, 81), dtype=torch.float64, device=dev)
             e = torch.zeros(0, dtype=torch.int64, device=dev)

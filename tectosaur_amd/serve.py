@@ -11,6 +11,11 @@ and cross to the host in one transfer, and the host-side Givens updates run.
 A converged (or failed) solve leaves the batch at once and its slot and
 memory are reused; requests arriving meanwhile join the next tick.
 
+Host-side set-up of arriving requests (mesh, pair classification, farfield
+layout, mass and constraint matrices) runs on a pool of worker threads while
+the device iterates the active solves; admission then only does the device
+half (pair integration, packing, uploads).
+
 Each solve's arithmetic does not depend on which other solves share its
 ticks (its kernels read only its own buffers), so a result is the same alone
 and in the mix.
@@ -18,13 +23,15 @@ and in the mix.
 ``serve_sequential`` is the one-at-a-time reference path (plain FP64 by
 default) the batched path is compared with.
 """
+import concurrent.futures
 import time
 
 import numpy as np
 import torch
 
 from .solver import ConstrainedOperator, make_state
-from .traction_slip import TractionSlipProblem, fault_mesh, request_traction
+from .traction_slip import (TractionSlipProblem, fault_mesh, prepare_problem,
+                            request_traction)
 
 
 class SolveRecord:
@@ -43,6 +50,7 @@ class SolveRecord:
         self.pair_evals = 0
         self.n_matvec = None
         self.setup_s = None
+        self.prep_s = None
 
     @property
     def latency(self):
@@ -52,7 +60,7 @@ class SolveRecord:
         return dict(rid=self.rid, n_tris=self.n_tris, arrival=self.arrival,
                     admitted=self.admitted, finished=self.finished, latency=self.latency,
                     status=self.status, iterations=self.iterations, restarts=self.restarts,
-                    rel_residual=self.rel_residual, setup_s=self.setup_s,
+                    rel_residual=self.rel_residual, setup_s=self.setup_s, prep_s=self.prep_s,
                     n_matvec=self.n_matvec)
 
 
@@ -77,11 +85,18 @@ class _Active:
         self.nbytes = nbytes
 
 
-def _build(req, mode, tol, restart, maxiter, orders):
+def _prepare(req, orders):
+    t0 = time.perf_counter()
     pts, tris = fault_mesh(req)
     t = request_traction(req, tris.shape[0])
+    prep = prepare_problem(pts, tris, t, orders=orders)
+    prep['prep_s'] = time.perf_counter() - t0
+    return prep
+
+
+def _build(prep, mode, tol, restart, maxiter):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
-    prob = TractionSlipProblem(pts, tris, t, orders=orders, farfield_precisions=precisions)
+    prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
     b = op.cmT.dot_dev(prob.rhs)
     state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
@@ -102,7 +117,7 @@ def _finalize(a, now, keep_slip):
 
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
-                 hbm_fraction=0.8, orders=None, keep_slip=False):
+                 hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4):
         self.mode = mode
         self.max_active = max_active
         self.tol = tol
@@ -111,76 +126,105 @@ class Server:
         self.hbm_fraction = hbm_fraction
         self.orders = orders
         self.keep_slip = keep_slip
+        self.prep_workers = prep_workers
         self.ticks = 0
 
     def run(self, requests, realtime=True, log=None):
         """Serve ``requests`` (sorted by arrival).  With realtime=True a
         request is admitted no earlier than its arrival time after start;
-        otherwise all are present at t = 0.  Returns SolveRecords."""
+        otherwise all are present at t = 0.  Admission is first-come
+        first-served.  Returns SolveRecords."""
         pending = sorted(requests, key=lambda r: r.arrival)
         recs = {}
         active = []
         used = 0
         free0, _ = torch.cuda.mem_get_info()
         budget = self.hbm_fraction * free0
+        pool = (concurrent.futures.ThreadPoolExecutor(self.prep_workers)
+                if self.prep_workers > 0 else None)
+        futures = {}
+        max_ahead = self.max_active + 2 * max(self.prep_workers, 1)
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
-        qi = 0
-        while qi < len(pending) or active:
-            now = clock()
-            # admission
-            while qi < len(pending) and len(active) < self.max_active:
-                req = pending[qi]
-                if realtime and req.arrival > now:
-                    break
-                need = estimate_bytes(req.n_tris, self.restart)
-                if active and used + need > budget:
-                    break
-                rec = SolveRecord(req)
-                if not realtime:
-                    rec.arrival = 0.0
-                rec.admitted = clock()
-                prob, op, state = _build(req, self.mode, self.tol, self.restart, self.maxiter,
-                                         self.orders)
-                rec.n_tris = prob.n_tris
-                rec.pair_evals = prob.pair_evals
-                rec.setup_s = clock() - rec.admitted
-                active.append(_Active(rec, prob, op, state, need))
-                recs[req.rid] = rec
-                used += need
-                qi += 1
+        qi = 0    # next request to admit
+        si = 0    # next request to hand to the preparation pool
+        try:
+            while qi < len(pending) or active:
                 now = clock()
-            if not active:
-                if qi < len(pending):
-                    time.sleep(max(0.0, min(0.01, pending[qi].arrival - clock())))
-                continue
-            # one tick: every active solve advances one step, one transfer
-            outs = [a.state.issue() for a in active]
-            sizes = [o.shape[0] for o in outs]
-            host = torch.cat(outs).cpu().numpy()
-            self.ticks += 1
-            off = 0
-            still = []
-            now = clock()
-            for a, s in zip(active, sizes):
-                done = a.state.consume(host[off:off + s])
-                off += s
-                if done:
-                    _finalize(a, now, self.keep_slip)
-                    used -= a.nbytes
-                    if log:
-                        log(a.rec)
-                else:
-                    still.append(a)
-            active = still
+                # host preparation of arrived requests, bounded look-ahead
+                while pool is not None and si < len(pending) and si < qi + max_ahead:
+                    if realtime and pending[si].arrival > now:
+                        break
+                    futures[pending[si].rid] = pool.submit(_prepare, pending[si], self.orders)
+                    si += 1
+                # admission (FIFO) of prepared requests
+                while qi < len(pending) and len(active) < self.max_active:
+                    req = pending[qi]
+                    if realtime and req.arrival > now:
+                        break
+                    need = estimate_bytes(req.n_tris, self.restart)
+                    if active and used + need > budget:
+                        break
+                    if pool is not None:
+                        fut = futures.get(req.rid)
+                        if fut is None or (active and not fut.done()):
+                            break
+                        prep = fut.result()
+                        del futures[req.rid]
+                    else:
+                        prep = None
+                    rec = SolveRecord(req)
+                    if not realtime:
+                        rec.arrival = 0.0
+                    rec.admitted = clock()
+                    if prep is None:
+                        prep = _prepare(req, self.orders)
+                    rec.prep_s = prep['prep_s']
+                    prob, op, state = _build(prep, self.mode, self.tol, self.restart,
+                                             self.maxiter)
+                    rec.n_tris = prob.n_tris
+                    rec.pair_evals = prob.pair_evals
+                    rec.setup_s = clock() - rec.admitted
+                    active.append(_Active(rec, prob, op, state, need))
+                    recs[req.rid] = rec
+                    used += need
+                    qi += 1
+                    now = clock()
+                if not active:
+                    if qi < len(pending):
+                        time.sleep(max(0.0, min(0.002, pending[qi].arrival - clock())))
+                    continue
+                # one tick: every active solve advances one step, one transfer
+                outs = [a.state.issue() for a in active]
+                sizes = [o.shape[0] for o in outs]
+                host = torch.cat(outs).cpu().numpy()
+                self.ticks += 1
+                off = 0
+                still = []
+                now = clock()
+                for a, sz in zip(active, sizes):
+                    done = a.state.consume(host[off:off + sz])
+                    off += sz
+                    if done:
+                        _finalize(a, now, self.keep_slip)
+                        used -= a.nbytes
+                        if log:
+                            log(a.rec)
+                    else:
+                        still.append(a)
+                active = still
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True, cancel_futures=True)
         torch.cuda.synchronize()
         self.makespan = clock()
         return [recs[r.rid] for r in pending]
 
 
 def serve_sequential(requests, mode='f64', realtime=True, **kw):
-    """The one-at-a-time path: each request solved alone, in arrival order."""
-    return Server(mode=mode, max_active=1, **kw).run(requests, realtime=realtime)
+    """The one-at-a-time path: each request prepared, built and solved alone,
+    in arrival order, with no overlap."""
+    return Server(mode=mode, max_active=1, prep_workers=0, **kw).run(requests, realtime=realtime)
 
 
 def summarize(recs, makespan):

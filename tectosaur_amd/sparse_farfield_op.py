@@ -30,13 +30,58 @@ from .kernels import kernels as KERNELS
 from .quadrature import gauss2d_tri
 
 
+def farfield_plan(nq_far, pts, tris, obs_subset, src_subset, curl):
+    """Host-side set-up of the operator (no device work): Morton order of
+    the triangles, quadrature points, normals, J w, curls, index maps."""
+    from .farfield import group_extent
+    pts = np.asarray(pts, dtype=np.float64)
+    tris = np.asarray(tris)
+    qx, qw = gauss2d_tri(nq_far)
+    nq = qx.shape[0]
+    phi = np.stack([1 - qx[:, 0] - qx[:, 1], qx[:, 0], qx[:, 1]], axis=1)
+
+    def side(subset):
+        v = pts[tris[subset]]
+        order = morton_order(v.mean(axis=1))
+        v = v[order]
+        un = np.cross(v[:, 2] - v[:, 0], v[:, 2] - v[:, 1])
+        jac = np.linalg.norm(un, axis=1)
+        n = un / jac[:, None]
+        qp = np.einsum('qb,tbd->tqd', phi, v).reshape((-1, 3))
+        qn = np.repeat(n, nq, axis=0)
+        wj = (jac[:, None] * qw[None, :]).reshape(-1)
+        return order, qp, qn, wj
+
+    def curls(subset):
+        # surface curls of the basis functions, DOF-space order
+        # (integral_primitives.cl:95-112): curl_b = (g_b0 e2 - g_b1 e1) / J
+        v = pts[tris[subset]]
+        e1 = v[:, 1] - v[:, 0]
+        e2 = v[:, 2] - v[:, 0]
+        J = np.linalg.norm(np.cross(v[:, 2] - v[:, 0], v[:, 2] - v[:, 1]), axis=1)[:, None]
+        c = np.stack([(-e2 + e1) / J, e2 / J, -e1 / J], axis=1)
+        return np.ascontiguousarray(c.reshape((-1, 9)))
+
+    o_order, o_pts, o_ns, o_wj = side(obs_subset)
+    s_order, s_pts, s_ns, s_wj = side(src_subset)
+    n_obs = obs_subset.shape[0]
+    pt_of_tri = np.empty(n_obs, dtype=np.int32)
+    pt_of_tri[o_order] = np.arange(n_obs, dtype=np.int32)
+    plan = dict(nq=nq, q=(qx, qw), phi=np.ascontiguousarray(phi), o_pts=o_pts, o_ns=o_ns,
+                o_wj=o_wj, s_pts=s_pts, s_ns=s_ns, s_wj=s_wj,
+                tri_of_pt=np.repeat(s_order, nq).astype(np.int32), pt_of_tri=pt_of_tri,
+                extents=(group_extent(o_pts), group_extent(s_pts)))
+    if curl:
+        plan['curl_src'] = curls(src_subset)
+        plan['curl_obs'] = curls(obs_subset)
+    return plan
+
+
 class TriToTriDirectFarfieldOp:
     def __init__(self, nq_far, K_name, params, pts, tris, float_type, obs_subset, src_subset,
-                 tile_cfg=None, precisions=None):
+                 tile_cfg=None, precisions=None, plan=None):
         K = KERNELS[K_name]
         self.curl = K.surf_curl_obs or K.surf_curl_src
-        pts = np.asarray(pts, dtype=np.float64)
-        tris = np.asarray(tris)
         obs_subset = np.asarray(obs_subset, dtype=np.int64)
         src_subset = np.asarray(src_subset, dtype=np.int64)
         self.shape = (obs_subset.shape[0] * 9, src_subset.shape[0] * 9)
@@ -47,55 +92,28 @@ class TriToTriDirectFarfieldOp:
         self.float_type = float_type
         self.prec = prec_of(float_type)
         self.K_name = K_name
-        self.q = gauss2d_tri(nq_far)
-        self.nq = self.q[0].shape[0]
-        qx, qw = self.q
-        phi = np.stack([1 - qx[:, 0] - qx[:, 1], qx[:, 0], qx[:, 1]], axis=1)
+        if plan is None:
+            plan = farfield_plan(nq_far, pts, tris, obs_subset, src_subset, self.curl)
+        self.q = plan['q']
+        self.nq = plan['nq']
         dev = N.require_device()
         self.device = dev
-
-        def side(subset):
-            t = tris[subset]
-            v = pts[t]
-            order = morton_order(v.mean(axis=1))
-            v = v[order]
-            un = np.cross(v[:, 2] - v[:, 0], v[:, 2] - v[:, 1])
-            jac = np.linalg.norm(un, axis=1)
-            n = un / jac[:, None]
-            qp = np.einsum('qb,tbd->tqd', phi, v).reshape((-1, 3))
-            qn = np.repeat(n, self.nq, axis=0)
-            wj = (jac[:, None] * qw[None, :]).reshape(-1)
-            return order, qp, qn, wj
-
-        def curls(subset):
-            # surface curls of the basis functions, DOF-space order
-            # (integral_primitives.cl:95-112): curl_b = (g_b0 e2 - g_b1 e1) / J
-            v = pts[tris[subset]]
-            e1 = v[:, 1] - v[:, 0]
-            e2 = v[:, 2] - v[:, 0]
-            J = np.linalg.norm(np.cross(v[:, 2] - v[:, 0], v[:, 2] - v[:, 1]), axis=1)[:, None]
-            c = np.stack([(-e2 + e1) / J, e2 / J, -e1 / J], axis=1)
-            return np.ascontiguousarray(c.reshape((-1, 9)))
-
-        o_order, o_pts, o_ns, o_wj = side(obs_subset)
-        s_order, s_pts, s_ns, s_wj = side(src_subset)
         if precisions is None:
             precisions = (self.prec,)
-        self.engine = FarfieldEngine(K_name, params, o_pts, o_ns, s_pts, s_ns,
-                                     precisions=precisions, device=dev, tile_cfg=tile_cfg)
+        self.engine = FarfieldEngine(K_name, params, plan['o_pts'], plan['o_ns'], plan['s_pts'],
+                                     plan['s_ns'], precisions=precisions, device=dev,
+                                     tile_cfg=tile_cfg, extents=plan['extents'])
         i32 = dict(dtype=torch.int32, device=dev)
         f64 = dict(dtype=torch.float64, device=dev)
-        # source point k belongs to DOF-space triangle s_order[k // nq]
-        self.tri_of_pt = torch.as_tensor(np.repeat(s_order, self.nq).astype(np.int32), **i32)
-        pt_of_tri = np.empty(self.n_obs, dtype=np.int32)
-        pt_of_tri[o_order] = np.arange(self.n_obs, dtype=np.int32)
-        self.pt_of_tri = torch.as_tensor(pt_of_tri, **i32)
-        self.qphi = torch.as_tensor(np.ascontiguousarray(phi), **f64)
-        self.wj_src = torch.as_tensor(s_wj, **f64)
-        self.wj_obs = torch.as_tensor(o_wj, **f64)
+        # source point k belongs to DOF-space triangle tri_of_pt[k]
+        self.tri_of_pt = torch.as_tensor(plan['tri_of_pt'], **i32)
+        self.pt_of_tri = torch.as_tensor(plan['pt_of_tri'], **i32)
+        self.qphi = torch.as_tensor(plan['phi'], **f64)
+        self.wj_src = torch.as_tensor(plan['s_wj'], **f64)
+        self.wj_obs = torch.as_tensor(plan['o_wj'], **f64)
         if self.curl:
-            self.curl_src = torch.as_tensor(curls(src_subset), **f64)
-            self.curl_obs = torch.as_tensor(curls(obs_subset), **f64)
+            self.curl_src = torch.as_tensor(plan['curl_src'], **f64)
+            self.curl_obs = torch.as_tensor(plan['curl_obs'], **f64)
 
     @property
     def pair_evals(self):

@@ -20,7 +20,7 @@ class RegularizedSparseIntegralOp:
     def __init__(self, nq_coincident, nq_edge_adj, nq_vert_adjacent, nq_far, nq_near,
                  near_threshold, K_near_name, K_far_name, params, pts, tris, float_type,
                  farfield_op_type=TriToTriDirectFarfieldOp, obs_subset=None, src_subset=None,
-                 farfield_precisions=None):
+                 farfield_precisions=None, near_plan=None, far_plan=None, keep_uncorrected=True):
         tris = np.asarray(tris)
         if obs_subset is None:
             obs_subset = np.arange(tris.shape[0])
@@ -28,11 +28,14 @@ class RegularizedSparseIntegralOp:
             src_subset = np.arange(tris.shape[0])
         self.nearfield = RegularizedNearfieldIntegralOp(
             pts, tris, obs_subset, src_subset, nq_coincident, nq_edge_adj, nq_vert_adjacent,
-            nq_far, nq_near, near_threshold, K_near_name, K_far_name, params, float_type)
-        if farfield_op_type is TriToTriDirectFarfieldOp and farfield_precisions is not None:
+            nq_far, nq_near, near_threshold, K_near_name, K_far_name, params, float_type,
+            plan=near_plan, keep_uncorrected=keep_uncorrected)
+        if farfield_op_type is TriToTriDirectFarfieldOp and (farfield_precisions is not None or
+                                                             far_plan is not None):
             self.farfield = TriToTriDirectFarfieldOp(nq_far, K_far_name, params, pts, tris,
                                                      float_type, obs_subset, src_subset,
-                                                     precisions=farfield_precisions)
+                                                     precisions=farfield_precisions,
+                                                     plan=far_plan)
         else:
             self.farfield = farfield_op_type(nq_far, K_far_name, params, pts, tris, float_type,
                                              obs_subset, src_subset)

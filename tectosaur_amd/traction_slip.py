@@ -118,26 +118,51 @@ def fault_mesh(req):
     return mesh_mod.make_rect(nx + 1, nz + 1, corners)
 
 
+def prepare_problem(pts, tris, traction, sm=3e10, orders=None, K_far='elasticRH3'):
+    """The host-only half of a traction -> slip set-up (mesh-side work that
+    needs no device): pair classification, farfield layout, mass matrix,
+    constraint matrix and right-hand side.  Safe to run on a worker thread
+    while the device is busy with other solves."""
+    from .nearfield_op import nearfield_plan
+    from .sparse_farfield_op import farfield_plan
+    from .kernels import kernels as KERNELS
+    o = dict(DEFAULT_ORDERS)
+    o.update(orders or {})
+    pts = np.asarray(pts, dtype=np.float64)
+    tris = np.asarray(tris, dtype=np.int64)
+    idx = np.arange(tris.shape[0])
+    K = KERNELS[K_far]
+    M = MassOp(o['quad_mass_order'], pts, tris)
+    t = np.asarray(traction, dtype=np.float64).reshape(-1)
+    return dict(pts=pts, tris=tris, orders=o, K_far=K_far,
+                near_plan=nearfield_plan(pts, tris, idx, idx, o['near_threshold']),
+                far_plan=farfield_plan(o['nq_far'], pts, tris, idx, idx,
+                                       K.surf_curl_obs or K.surf_curl_src),
+                M=M, cm=continuity_cm(tris), rhs=-M.dot(t / sm))
+
+
 class TractionSlipProblem:
     """Device operators and right-hand side of one traction -> slip solve."""
 
     def __init__(self, pts, tris, traction, sm=3e10, nu=0.25, orders=None,
-                 farfield_precisions=('f32', 'f64'), K_far='elasticRH3'):
-        o = dict(DEFAULT_ORDERS)
-        o.update(orders or {})
-        self.pts = np.asarray(pts, dtype=np.float64)
-        self.tris = np.asarray(tris, dtype=np.int64)
+                 farfield_precisions=('f32', 'f64'), K_far='elasticRH3', prepared=None):
+        if prepared is None:
+            prepared = prepare_problem(pts, tris, traction, sm, orders, K_far)
+        o = prepared['orders']
+        self.pts = prepared['pts']
+        self.tris = prepared['tris']
         n = self.tris.shape[0]
         self.n_tris = n
         self.H = RegularizedSparseIntegralOp(
             o['nq_coincident'], o['nq_edge_adj'], o['nq_vert_adjacent'], o['nq_far'],
-            o['nq_near'], o['near_threshold'], 'elasticRH3', K_far, [1.0, nu], self.pts,
-            self.tris, np.float32, farfield_precisions=farfield_precisions)
+            o['nq_near'], o['near_threshold'], 'elasticRH3', prepared['K_far'], [1.0, nu],
+            self.pts, self.tris, np.float32, farfield_precisions=farfield_precisions,
+            near_plan=prepared['near_plan'], far_plan=prepared['far_plan'],
+            keep_uncorrected=False)
         self.device = self.H.device
-        self.M = MassOp(o['quad_mass_order'], self.pts, self.tris)
-        self.cm = continuity_cm(self.tris)
-        t = np.asarray(traction, dtype=np.float64).reshape(-1)
-        self.rhs = torch.as_tensor(-self.M.dot(t / sm), dtype=torch.float64, device=self.device)
+        self.M = prepared['M']
+        self.cm = prepared['cm']
+        self.rhs = torch.as_tensor(prepared['rhs'], dtype=torch.float64, device=self.device)
 
     @property
     def n_dofs(self):
