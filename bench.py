@@ -27,7 +27,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (datasheet)
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector peak (datasheet)
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E bandwidth (datasheet)
 PARAMS = [3e10, 0.25]
+# Algorithmic FP64 flops of one RH3 quadrature-point evaluation in the pair
+# integrator: two P1 maps (2 x 3 coords x 5) + D (3) + basis weights (4)
+# + r^2 (5) + rsqrt (1) + w/r, w/r^3 (3) + 1/r moment (1) + six D_p D_t / r^3
+# moments (6 x 3) = 65.
+RH_NEAR_FLOPS_PER_QP = 65
+# HBM bytes per launch from hardware counters (rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE in separate runs, scaled x2 and x1 by a 1 GiB copy measured in
+# the same runs): profiles/far_p2p_H_two_surface_442k_pmc_traffic.md.
+TRAFFIC_FAR_P2P_H_F32 = 2.389e9
+TRAFFIC_BSR_ICOSPHERE_L5 = 1.345e9
+TRAFFIC_SOURCE = 'profiles/far_p2p_H_two_surface_442k_pmc_traffic.md'
 
 
 def cfg2_mesh():
@@ -66,6 +79,90 @@ def kernel_event_time(op, prec, reps):
         b.synchronize()
         t.append(a.elapsed_time(b) * 1e-3)
     return float(np.mean(t))
+
+
+def stream_event_time(fn, reps):
+    """Average wall time of ``fn()`` on torch's current stream, from HIP
+    events recorded on that stream around each call."""
+    import torch
+    s = torch.cuda.current_stream()
+    t = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        fn()
+        b.record(s)
+        b.synchronize()
+        t.append(a.elapsed_time(b) * 1e-3)
+    return float(np.mean(t))
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size (reads nbytes, writes nbytes), run
+    so that hardware byte counters (FETCH_SIZE, WRITE_SIZE) read in the same
+    process can be scaled to bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    dt = stream_event_time(lambda: dst.copy_(src), reps)
+    del src, dst
+    torch.cuda.empty_cache()
+    return {'bytes_read': nbytes, 'bytes_written': nbytes, 'ms': dt * 1e3,
+            'GB_per_s': 2 * nbytes / dt / 1e9}
+
+
+def nearfield_bench(level=5, nq=8, thr=3.0, reps=20):
+    """Nearfield on the icosphere workload: level-5 icosphere (20,480
+    triangles, unit radius), every rule of order 8, near threshold 3, RH3
+    near with the H3 farfield correction, FP64.  Reports the pair
+    integrator's rate on the plain-near class (the largest) and the BSR
+    matvec (the per-iteration nearfield gather) against HBM bandwidth."""
+    import torch
+    from tectosaur_amd import mesh
+    from tectosaur_amd.nearfield_op import RegularizedNearfieldIntegralOp
+    from tectosaur_amd.pairs_integrator import PairsIntegrator
+    pts, tris = mesh.make_icosphere([0, 0, 0], 1.0, level)
+    n = tris.shape[0]
+    idx = np.arange(n)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op = RegularizedNearfieldIntegralOp(pts, tris, idx, idx, nq, nq, nq, 2, nq, thr, 'elasticRH3',
+                                        'elasticH3', PARAMS, np.float32, keep_classes=True,
+                                        keep_uncorrected=False)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    prs = op.classes['near'][0]
+    pi = PairsIntegrator('elasticRH3', PARAMS, np.float32, 2, nq, pts, tris)
+    p_dev = torch.as_tensor(np.ascontiguousarray(prs.astype(np.int32)), device='cuda')
+    out = torch.empty((prs.shape[0], 81), dtype=torch.float64, device='cuda')
+    nqp = pi.near_q[0].shape[0]
+    kt = stream_event_time(lambda: pi.pairs_quad_dev(pi.near_q, p_dev, False, out), 3)
+    qps = prs.shape[0] * nqp / kt
+    op.classes = {}
+    del out, p_dev
+    torch.cuda.empty_cache()
+    mat = op.mat
+    x = torch.as_tensor(np.random.default_rng(0).standard_normal(mat.shape[1]), device='cuda')
+    y = torch.empty(mat.shape[0], dtype=torch.float64, device='cuda')
+    mat.dot_dev(x, y)
+    bt = stream_event_time(lambda: mat.dot_dev(x, y), reps)
+    gbs = mat.algorithmic_bytes / bt / 1e9
+    res = {'workload': 'icosphere level %d (%d tris), rules of order %d, near threshold %g, '
+                       'RH3 near / H3 correction, FP64' % (level, n, nq, thr),
+           'pair_counts': op.counts, 'build_s': t_build,
+           'pairs_kernel': {'class': 'near', 'pairs': int(prs.shape[0]), 'qp_per_pair': nqp,
+                            'ms': kt * 1e3, 'qp_evals_per_s': qps,
+                            'flops_per_qp': RH_NEAR_FLOPS_PER_QP,
+                            'tflops': qps * RH_NEAR_FLOPS_PER_QP / 1e12,
+                            'frac_fp64_peak': qps * RH_NEAR_FLOPS_PER_QP / 1e12 /
+                            PEAK_FP64_TFLOPS},
+           'bsr_matvec': {'blocks': int(mat.nnzb), 'bytes': int(mat.algorithmic_bytes),
+                          'ms': bt * 1e3, 'GB_per_s': gbs, 'frac_hbm_peak': gbs / PEAK_HBM_GBS,
+                          'traffic': TRAFFIC_BSR_ICOSPHERE_L5 if level == 5 and nq == 8 else None,
+                          'traffic_source': TRAFFIC_SOURCE}}
+    print('# nearfield: %s' % json.dumps(res), flush=True)
+    return res
 
 
 def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
@@ -130,9 +227,12 @@ def main():
     ap.add_argument('--cpu-rows', type=int, default=4096)
     ap.add_argument('--kernels', default='elasticU3,elasticT3,elasticA3,elasticH3')
     ap.add_argument('--serve-requests', type=int, default=48)
-    ap.add_argument('--serve-max-tris', type=int, default=8192)
-    ap.add_argument('--serve-rate', type=float, default=4.0)
+    ap.add_argument('--serve-max-tris', type=int, default=32768)
+    ap.add_argument('--serve-rate', type=float, default=100.0)
     ap.add_argument('--no-serve', action='store_true')
+    ap.add_argument('--no-nearfield', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='run a 1 GiB device copy first (scale for byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
@@ -142,6 +242,7 @@ def main():
     import torch
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
     torch.cuda.set_device(0)
+    calib = copy_calibration() if args.copy_calibration else None
     t_setup = time.perf_counter()
     pts, tris = cfg2_mesh()
     n = tris.shape[0]
@@ -203,11 +304,17 @@ def main():
                    'n_tris': int(n), 'n_points': int(n * 3), 'pair_evals_per_step': int(pe),
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS,
-                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32_TFLOPS, 'traffic': None,
+                     'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32_TFLOPS,
+                     'traffic': TRAFFIC_FAR_P2P_H_F32, 'traffic_source': TRAFFIC_SOURCE,
+                     'algorithmic_bytes': int(n * 3 * (24 + 48 + 24 * H_op.engine.tile_cfg['f32'].src_chunks)),
                      'kernel': 'far_p2p_f32_kernel<H>', 'kernel_ms': kt * 1e3,
                      'flops_per_pair': 102},
         'kernels': per_kernel,
     }
+    if calib is not None:
+        line['copy_calibration'] = calib
+    if not args.no_nearfield:
+        line['nearfield'] = nearfield_bench()
     if not args.no_serve:
         line['served'] = serve_bench(args.serve_requests, args.serve_max_tris, args.serve_rate)
     if not args.no_cpu_baseline:
