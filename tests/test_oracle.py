@@ -170,7 +170,7 @@ def test_farfield_pts_bruteforce(oracle):
         np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize('K', KS)
+@pytest.mark.parametrize('K', KS + ['elasticRT3', 'elasticRA3', 'elasticRH3'])
 def test_farfield_tris_equals_dense_pairs(oracle, K):
     # a2 (matrix-free, vector code) == the dense pair integral (tensor code)
     # summed against the input, on a small separated mesh pair.
@@ -187,3 +187,43 @@ def test_farfield_tris_equals_dense_pairs(oracle, K):
     blocks = oracle.pairs_integrate(K, qx4, qw4, pts, tris, pairs, True, [1.0, 0.25])
     dense = blocks.reshape(n, n, 9, 9).transpose(0, 2, 1, 3).reshape(9 * n, 9 * n)
     np.testing.assert_allclose(out, dense @ v, rtol=1e-10, atol=1e-10 * np.abs(out).max())
+
+
+def _two_spheres():
+    m1 = mesh.make_icosphere([0, 0, 0], 1.0, 1)
+    m2 = mesh.make_icosphere([2.6, 0.4, -0.3], 1.0, 1)
+    pts = np.vstack([m1[0], m2[0]])
+    tris = np.vstack([m1[1], m2[1] + m1[0].shape[0]])
+    n1 = m1[1].shape[0]
+    return pts, tris[:n1], tris[n1:]
+
+
+def test_regularized_kernels_equal_singular_forms_on_closed_surfaces(oracle):
+    # Integration by parts on closed surfaces (no boundary terms): for a
+    # continuous P1 source field, RT = T (source side regularised); with a
+    # continuous observation test space (cm^T sums the DOFs at a vertex),
+    # RA = A for any source field and RH = H for a continuous one.  Two
+    # disjoint unit icospheres, so every integrand is smooth and the
+    # difference is quadrature error only: it must fall spectrally with the
+    # rule order.  A wrong sign, constant or curl index in RT/RA/RH gives an
+    # O(1) difference (as dropping the continuity of the test space does).
+    from tectosaur_amd.traction_slip import continuity_cm
+    pts, ot, st = _two_spheres()
+    f = lambda p: np.stack([np.sin(1.3 * p[..., 0] + 0.2), np.cos(0.7 * p[..., 1]) * p[..., 2],
+                            0.5 * p[..., 2] ** 2 - p[..., 0]], axis=-1)
+    v = f(pts[st]).reshape(-1)
+    vr = np.random.default_rng(3).normal(size=v.shape)
+    cmT = continuity_cm(ot).T
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    errs = {}
+    for nq in (3, 6):
+        qx, qw = gauss2d_tri(nq)
+        ff = lambda K, x: oracle.farfield_tris(K, qx, qw, pts, ot, st, x, [1.0, 0.25])
+        errs[nq] = (rel(ff('elasticRT3', v), ff('elasticT3', v)),
+                    rel(cmT @ ff('elasticRA3', vr), cmT @ ff('elasticA3', vr)),
+                    rel(cmT @ ff('elasticRH3', v), cmT @ ff('elasticH3', v)))
+        if nq == 6:
+            assert rel(ff('elasticRH3', v), ff('elasticH3', v)) > 0.1
+    for k in range(3):
+        assert errs[6][k] < 2e-9, errs
+        assert errs[6][k] < 1e-3 * errs[3][k], errs
