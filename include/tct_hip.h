@@ -161,6 +161,45 @@ int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const int32_t *pt_o
                          int64_t n_obs_pad, int chunks, const double *partial,
                          const double *params_host, double *out, void *stream);
 
+/* ---------------- treecode farfield ----------------------------------------
+ * Reference: the treecode TSFMM (tectosaur/fmm/tsfmm.py:30-264: p2m :160-177,
+ * m2m :178-192, m2p :193-214, p2p :215-240) with the elasticU3/RT3/RA3/RH3
+ * expansions of tectosaur/fmm/ts_kernels.cl, used through FMMFarfieldOp
+ * (tectosaur/ops/sparse_farfield_op.py:73-118).  Own formulation: Cartesian
+ * Taylor moments of 1/r, per-kernel Laplace charges, scaled by the node
+ * radius.  The tree is a binary tree over the 64-point Morton tiles of the
+ * packed source set (tct_far_pack_src_ex layout): node t < n_tiles is tile
+ * t; parents follow their children.
+ *   node_cr [n_nodes][4] f64: centre (3) and radius (> 0) of every node;
+ *   moments [n_nodes][C][n_terms(order)] in the precision of the records,
+ *     C = 4 (U, RA), 13 (RT), 12 (RH), n_terms(p) = (p+1)(p+2)(p+3)/6.
+ * Kernels: TCT_U, TCT_RT, TCT_RA, TCT_RH; orders 0..TCT_TREE_MAX_ORDER for
+ * p2m/m2m, and 2-6 for eval (others return TCT_EUNSUPPORTED). */
+#define TCT_TREE_MAX_ORDER 8
+
+/* Tile moments (nodes 0..n_tiles-1) from the packed source records (the
+ * first n_src are real points, the rest padding): FP32 records are local to
+ * src_org per tile, FP64 records absolute (src_org unused). */
+int tct_tree_p2m(int kernel, int prec, int order, int64_t n_src, int64_t n_tiles, const void *rec,
+                 const double *src_org, const double *node_cr, void *moments, void *stream);
+
+/* One upward level: parents [p_off, p_off + n_par) from children
+ * c_off + 2i and c_off + 2i + 1 (the latter when 2i + 1 < n_child). */
+int tct_tree_m2m(int kernel, int prec, int order, int p_off, int n_par, int c_off, int n_child,
+                 const double *node_cr, void *moments, void *stream);
+
+/* Evaluation over per-observation-tile interaction lists: list_start
+ * [n_obs_tiles + 1] int32, list int32 entries e >= 0 (m2p from node e) or
+ * e < 0 (p2p with source tile -(e + 1)).  obs = loc32 (FP32, with obs_org)
+ * or soa64 (FP64); rec/src_org as for tct_far_p2p.  Each list is split into
+ * `chunks` parts; out_partial [chunks][3 or 9][n_obs_pad] as tct_far_p2p,
+ * so tct_far_reduce_pts / tct_far_scatter_tris / tct_far_scatter_curl apply. */
+int tct_tree_eval(int kernel, int prec, int order, int n_obs_tiles, int n_obs_pad,
+                  const void *obs, const double *obs_org, const void *rec, const double *src_org,
+                  const double *node_cr, const void *moments, const int *list_start,
+                  const int *list, const double *params_host, double eps2, int chunks,
+                  double *out_partial, void *stream);
+
 /* ---------------- nearfield pair integrator (FP64) ----------------------
  * Reference: single_pairs{Z,N} / single_pairsN_adj (tectosaur/assemble.cl
  * :24-103) driven by PairsIntegrator (tectosaur/nearfield/

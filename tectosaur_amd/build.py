@@ -14,7 +14,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libtct_hip.so')
-HIP_SOURCES = ['abi.hip', 'far_p2p.hip', 'pairs.hip', 'sparse.hip']
+HIP_SOURCES = ['abi.hip', 'far_p2p.hip', 'far_tree.hip', 'far_tree_eval_u.hip',
+               'far_tree_eval_rt.hip', 'far_tree_eval_ra.hip', 'far_tree_eval_rh.hip', 'pairs.hip',
+               'sparse.hip']
 HOST_SOURCES = ['host_mesh.cpp', 'host_constraints.cpp']
 ARCH = 'gfx950'
 

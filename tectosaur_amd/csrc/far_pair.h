@@ -1,0 +1,188 @@
+// Shared pieces of the farfield kernels (direct N-body and treecode):
+// kernel constants, the masked reciprocal square root, source-record
+// sizes and the per-pair kernel bodies.
+#pragma once
+#include "tct_common.h"
+
+namespace tct {
+
+constexpr int G = TCT_GROUP;  // points per origin group / source tile
+constexpr int R = TCT_REC;    // words per source record
+
+// Kernel constants after folding the leading constant into the strengths.
+// eps2: pairs with r2 <= eps2 are skipped.  The FP64 path uses 0 (the
+// reference's exact r2 == 0 test).  The FP32 path uses a bound on the
+// rounding of a point stored relative to two different group origins, so a
+// point paired with itself is skipped even when obs and src are packed
+// differently; real point spacings are many orders of magnitude larger.
+template <typename T>
+struct KConst {
+    T c0, c1, c2, c3, c4, eps2;
+};
+
+__device__ __forceinline__ float rsq_masked(float r2, float eps2) {
+    float v = __builtin_amdgcn_rsqf(r2);
+    return r2 > eps2 ? v : 0.0f;
+}
+__device__ __forceinline__ double rsq_masked(double r2, double eps2) {
+    double v = rsqrt(r2);
+    return r2 > eps2 ? v : 0.0;
+}
+
+template <int KID>
+constexpr int rec_words() { return KID == TCT_RH ? 24 : 12; }
+template <int KID>
+constexpr int n_out() { return (KID == TCT_RA || KID == TCT_RH) ? 9 : 3; }
+template <int KID>
+constexpr bool needs_obs_normal() { return KID == TCT_A || KID == TCT_H || KID == TCT_RA; }
+
+// One (observation point, source point) pair, accumulate into acc[n_out].
+// x: observation point (already shifted into the source tile frame);
+// m: observation unit normal; q: the source record (wave-uniform).
+//   U/T/A/H: q = [y n s e0 e1 -], s the strength scaled by the kernel
+//            constant, e0/e1 per-source dot products;
+//   RT:      q = [y n s e], s = CsRT2 w J in(q), e = CsRT1 w J a with
+//            a_t = sum_{d,I} eps_{I t d} S_{dI} (S the source surface curl);
+//   RA:      q = [y - s -], s = w J in(q);
+//   RH:      q = [y G(9) P(9) -], G = w J CsRH0 S,
+//            P = w J CsRH0 (CsRH1 tr(S) I + CsRH2 S^T - 2 S).
+template <int KID, typename T>
+__device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *__restrict__ q,
+                                          const KConst<T> &k, T *acc) {
+    const T D0 = q[0] - x[0], D1 = q[1] - x[1], D2 = q[2] - x[2];
+    const T r2 = D0 * D0 + D1 * D1 + D2 * D2;
+    const T invr = rsq_masked(r2, k.eps2);
+    const T w = invr * invr;
+    if constexpr (KID == TCT_U) {
+        // U: (3-4nu)/r s + (D.s) D / r^3, strengths scaled by CsU1.
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
+        const T ds = D0 * s0 + D1 * s1 + D2 * s2;
+        const T a = k.c0 * invr;
+        const T b = invr * w * ds;
+        acc[0] += a * s0 + b * D0;
+        acc[1] += a * s1 + b * D1;
+        acc[2] += a * s2 + b * D2;
+    } else if constexpr (KID == TCT_T) {
+        // T: A(-(n.D) s + (n x D) x s) - C (n.D)(D.s) D with
+        // (n x D) x s = D (n.s) - n (D.s); strengths scaled by CsT1,
+        // k.c0 = CsT0 / CsT1, e0 = n.s.
+        const T n0 = q[3], n1 = q[4], n2 = q[5];
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
+        const T q3 = invr * w;
+        const T al = k.c0 * q3;
+        const T rn = n0 * D0 + n1 * D1 + n2 * D2;
+        const T ds = D0 * s0 + D1 * s1 + D2 * s2;
+        const T be = -al * rn;
+        const T ga = -al * ds;
+        const T de = al * q[9] - q3 * w * rn * ds;
+        acc[0] += be * s0 + ga * n0 + de * D0;
+        acc[1] += be * s1 + ga * n1 + de * D1;
+        acc[2] += be * s2 + ga * n2 + de * D2;
+    } else if constexpr (KID == TCT_A) {
+        // A: the adjoint form with the observation normal m.
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
+        const T q3 = invr * w;
+        const T al = k.c0 * q3;
+        const T rn = m[0] * D0 + m[1] * D1 + m[2] * D2;
+        const T ds = D0 * s0 + D1 * s1 + D2 * s2;
+        const T sm = m[0] * s0 + m[1] * s1 + m[2] * s2;
+        const T be = al * rn;
+        const T ga = -al * ds;
+        const T de = al * sm + q3 * w * rn * ds;
+        acc[0] += be * s0 + ga * m[0] + de * D0;
+        acc[1] += be * s1 + ga * m[1] + de * D1;
+        acc[2] += be * s2 + ga * m[2] + de * D2;
+    } else if constexpr (KID == TCT_H) {
+        // H with strengths scaled by CsH0; k = (CsH1, CsH3, 3 CsH1, nu, 3 nu),
+        // e0 = CsH2 (n.s), e1 = 3 CsH1 (n.s).
+        const T n0 = q[3], n1 = q[4], n2 = q[5];
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
+        const T Q = invr * w;
+        const T a = n0 * D0 + n1 * D1 + n2 * D2;
+        const T b = m[0] * D0 + m[1] * D1 + m[2] * D2;
+        const T c = s0 * D0 + s1 * D1 + s2 * D2;
+        const T sm = s0 * m[0] + s1 * m[1] + s2 * m[2];
+        const T mn = m[0] * n0 + m[1] * n1 + m[2] * n2;
+        const T wa = w * a;
+        const T wbc = w * b * c;
+        const T MT = Q * (q[9] + k.c2 * wa * c);
+        const T NT = Q * (k.c0 * sm + k.c1 * wbc);
+        const T inner = k.c3 * sm - T(5) * wbc;
+        const T DT = Q * w * (q[10] * b + k.c1 * c * mn + T(3) * a * inner);
+        const T ST = Q * (k.c4 * wa * b + k.c0 * mn);
+        acc[0] += n0 * NT + m[0] * MT + D0 * DT + s0 * ST;
+        acc[1] += n1 * NT + m[1] * MT + D1 * DT + s1 * ST;
+        acc[2] += n2 * NT + m[2] * MT + D2 * DT + s2 * ST;
+    } else if constexpr (KID == TCT_RT) {
+        // RT: invr e + (CsRT0/CsRT1) invr^3 (D.e) D + invr^3 (n.D) s
+        const T i3 = invr * w;
+        const T De = D0 * q[9] + D1 * q[10] + D2 * q[11];
+        const T nD = D0 * q[3] + D1 * q[4] + D2 * q[5];
+        const T cd = k.c0 * i3 * De;
+        const T cs = i3 * nD;
+        acc[0] += invr * q[9] + cd * D0 + cs * q[6];
+        acc[1] += invr * q[10] + cd * D1 + cs * q[7];
+        acc[2] += invr * q[11] + cd * D2 + cs * q[8];
+    } else if constexpr (KID == TCT_RA) {
+        // RA moments: v1 += s / r, v2 += (D.s) D / r^3, v3 += (m.D) s / r^3;
+        // the constants and the observation curls are applied by the scatter.
+        const T s0 = q[6], s1 = q[7], s2 = q[8];
+        const T i3 = invr * w;
+        const T Ds = i3 * (D0 * s0 + D1 * s1 + D2 * s2);
+        const T mD = i3 * (D0 * m[0] + D1 * m[1] + D2 * m[2]);
+        acc[0] += invr * s0;
+        acc[1] += invr * s1;
+        acc[2] += invr * s2;
+        acc[3] += Ds * D0;
+        acc[4] += Ds * D1;
+        acc[5] += Ds * D2;
+        acc[6] += mD * s0;
+        acc[7] += mD * s1;
+        acc[8] += mD * s2;
+    } else {
+        // RH: M[k][m] += P[k][m] / r - 2 D_k (D^T G)_m / r^3
+        const T Dk[3] = {D0, D1, D2};
+        const T qd = T(-2) * invr * w;
+        T u[3];
+#pragma unroll
+        for (int mm = 0; mm < 3; ++mm) u[mm] = D0 * q[3 + mm] + D1 * q[6 + mm] + D2 * q[9 + mm];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            const T dq = qd * Dk[kk];
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+                acc[kk * 3 + mm] += invr * q[12 + kk * 3 + mm] + dq * u[mm];
+        }
+    }
+}
+
+
+template <typename T>
+inline KConst<T> kernel_consts(int kid, const double *params, double eps2) {
+    ElasticConsts c = elastic_consts(params);
+    KConst<T> k{};
+    k.eps2 = (T)eps2;
+    switch (kid) {
+    case TCT_U: k.c0 = (T)(c.CsU0 / c.CsU1); break;
+    case TCT_T:
+    case TCT_A: k.c0 = (T)(c.CsT0 / c.CsT1); break;
+    case TCT_H:
+        k.c0 = (T)c.CsH1;
+        k.c1 = (T)c.CsH3;
+        k.c2 = (T)(3.0 * c.CsH1);
+        k.c3 = (T)c.nu;
+        k.c4 = (T)(3.0 * c.nu);
+        break;
+    case TCT_RT: k.c0 = (T)(c.CsRT0 / c.CsRT1); break;
+    case TCT_RH:
+        // used by the treecode's RH expansion only (the point form folds
+        // every constant into the source record)
+        k.c0 = (T)c.CsRH1;
+        k.c1 = (T)c.CsRH2;
+        break;
+    default: break;
+    }
+    return k;
+}
+
+}  // namespace tct

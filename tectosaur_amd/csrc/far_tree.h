@@ -1,0 +1,506 @@
+// Treecode templates shared by far_tree.hip (p2m, m2m, ABI) and the
+// per-kernel evaluation units far_tree_eval_{u,rt,ra,rh}.hip.
+#pragma once
+// Treecode farfield for the elastic kernels U, RT, RA, RH on gfx950.
+//
+// Reference: the treecode of tectosaur/fmm/tsfmm.py (p2p, p2m, m2m, m2p
+// passes over an octree; acceptance test of tectosaur/fmm/traversal.cpp:124-
+// 160) with the kernels of tectosaur/fmm/ts_kernels.cl (elasticU3, RT3, RA3,
+// RH3).  The result approximates the direct sum of far_p2p.hip; its error
+// falls with the expansion order and the acceptance parameter.
+//
+// Formulation (own design): every supported kernel is a combination of
+// Laplace potentials and their gradients at the observation point, so one
+// Cartesian Taylor expansion of 1/r serves all of them.  With D = y - x,
+// 1/r = 1/|D| and d/dx_k (1/r) = D_k / r^3:
+//   sum_j (D.f_j) D_k / r^3 = d_k Psi - x'_b d_k Phi_b,
+//   Phi_b = sum_j f_jb / r,   Psi = sum_j (y'_j . f_j) / r,
+// with y' = y - c and x' = x - c relative to the expansion centre c.
+// Channels (Laplace charges per source point, from the source record):
+//   U, RA: s (3), y'.s (1);
+//   RT:    e (3), y'.e (1), s_a n_j (9);
+//   RH:    G_jm (9), y'.G_:m (3).
+// Outputs are assembled per kernel from Phi, Psi and their gradients (see
+// m2p_eval).  Moments of a node with centre c and radius rho are stored
+// scaled, M_a = sum q ((y - c)/rho)^a, and evaluated with the derivatives
+// a_a(u) = (1/a!) d^a (1/|u|) at u = (c - x)/rho, so every stored and
+// computed number is O(1) in FP32 whatever the mesh units.
+//
+// Layout: the tree is a binary tree over the 64-point Morton tiles of the
+// packed source set (far_p2p.hip's layout; a node covers a contiguous range
+// of tiles).  Interaction lists are per observation tile (one wave): entry
+// e >= 0 is an m2p node, e < 0 is a p2p source tile -(e + 1).  Each wave
+// walks its list with wave-uniform loads (node moments and source records
+// come through the scalar cache).  grid.y splits each list into chunks
+// whose partial sums are added in fixed order by the reduce/scatter step.
+#include <type_traits>
+#include <utility>
+
+#include "far_pair.h"
+
+namespace tct {
+
+// ---------------------------------------------------------------------------
+// Multi-indices in graded order: degree n ascending; within a degree, the
+// x exponent descending, then the y exponent descending.
+__host__ __device__ constexpr int n_terms(int p) {
+    return p < 0 ? 0 : (p + 1) * (p + 2) * (p + 3) / 6;
+}
+__host__ __device__ constexpr int mi_pos(int a, int b, int c) {
+    return n_terms(a + b + c - 1) + (b + c) * (b + c + 1) / 2 + c;
+}
+struct Mi {
+    int a, b, c;
+};
+__host__ __device__ constexpr Mi mi_at(int i) {
+    int n = 0;
+    while (n_terms(n) <= i) ++n;
+    int r = i - n_terms(n - 1);
+    int k = 0;  // k = n - a
+    while ((k + 1) * (k + 2) / 2 <= r) ++k;
+    const int c = r - k * (k + 1) / 2;
+    return Mi{n - k, k - c, c};
+}
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// a_a(u) = (1/a!) d^a (1/|u|) for |a| <= P, by the recurrence
+// n |u|^2 a_a = -(2n - 1) sum_k u_k a_{a-e_k} - (n - 1) sum_k a_{a-2e_k}.
+template <int P, typename T>
+__device__ __forceinline__ void taylor_derivs(const T u[3], T *a) {
+    const T u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const T inv_u2 = T(1) / u2;
+    a[0] = sqrt(inv_u2);
+    const T w[3] = {u[0] * inv_u2, u[1] * inv_u2, u[2] * inv_u2};
+    static_for<1, n_terms(P)>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        constexpr Mi m = mi_at(I);
+        constexpr int n = m.a + m.b + m.c;
+        constexpr T c1 = T(2 * n - 1) / T(n);
+        constexpr T c2 = T(n - 1) / T(n);
+        T s1 = T(0), s2 = T(0);
+        if constexpr (m.a >= 1) s1 += w[0] * a[mi_pos(m.a - 1, m.b, m.c)];
+        if constexpr (m.b >= 1) s1 += w[1] * a[mi_pos(m.a, m.b - 1, m.c)];
+        if constexpr (m.c >= 1) s1 += w[2] * a[mi_pos(m.a, m.b, m.c - 1)];
+        if constexpr (m.a >= 2) s2 += a[mi_pos(m.a - 2, m.b, m.c)];
+        if constexpr (m.b >= 2) s2 += a[mi_pos(m.a, m.b - 2, m.c)];
+        if constexpr (m.c >= 2) s2 += a[mi_pos(m.a, m.b, m.c - 2)];
+        if constexpr (n >= 2)
+            a[I] = -(c1 * s1 + c2 * inv_u2 * s2);
+        else
+            a[I] = -(c1 * s1);
+    });
+}
+
+// Channel layout per kernel.
+template <int KID>
+struct TreeK;
+template <>
+struct TreeK<TCT_U> {
+    static constexpr int C = 4;
+};
+template <>
+struct TreeK<TCT_RA> {
+    static constexpr int C = 4;
+};
+template <>
+struct TreeK<TCT_RT> {
+    static constexpr int C = 13;
+};
+template <>
+struct TreeK<TCT_RH> {
+    static constexpr int C = 12;
+};
+
+// Laplace charge of channel ch for one source record q (yp = y - c, physical).
+template <int KID, typename T>
+__device__ __forceinline__ double charge(int ch, const T *q, const double yp[3]) {
+    if constexpr (KID == TCT_U || KID == TCT_RA) {
+        if (ch < 3) return (double)q[6 + ch];
+        return yp[0] * q[6] + yp[1] * q[7] + yp[2] * q[8];
+    } else if constexpr (KID == TCT_RT) {
+        if (ch < 3) return (double)q[9 + ch];
+        if (ch == 3) return yp[0] * q[9] + yp[1] * q[10] + yp[2] * q[11];
+        const int a = (ch - 4) / 3, j = (ch - 4) % 3;
+        return (double)q[6 + a] * (double)q[3 + j];
+    } else {
+        if (ch < 9) return (double)q[3 + ch];
+        const int m = ch - 9;
+        return yp[0] * q[3 + m] + yp[1] * q[6 + m] + yp[2] * q[9 + m];
+    }
+}
+
+// The y'-dependent channels: Psi channel index and its three source
+// channels (Psi = sum_b y'_b q_b), for the shift of centre in m2m.
+template <int KID>
+__device__ __forceinline__ int psi_count() {
+    return KID == TCT_RH ? 3 : 1;
+}
+template <int KID>
+__device__ __forceinline__ int psi_channel(int i) {
+    return KID == TCT_RH ? 9 + i : 3;
+}
+template <int KID>
+__device__ __forceinline__ int psi_source(int i, int b) {
+    return KID == TCT_RH ? 3 * b + i : b;
+}
+
+// ---------------------------------------------------------------------------
+// Tile moments (tree level 0).  grid = n_tiles, block = 256; thread o
+// computes output (channel, term) o, looping over the tile's 64 records.
+// node_cr[4 * node] = centre (3) and radius; tile t is node t.
+template <int KID, typename T, bool ABS>
+__global__ __launch_bounds__(256) void tree_p2m_kernel(int order, int64_t n_src,
+                                                       const T *__restrict__ rec,
+                                                       const double *__restrict__ src_org,
+                                                       const double *__restrict__ node_cr,
+                                                       T *__restrict__ moments) {
+    constexpr int C = TreeK<KID>::C;
+    constexpr int RW = rec_words<KID>();
+    const int t = blockIdx.x;
+    const int TT = n_terms(order);
+    const double *cr = node_cr + 4 * (size_t)t;
+    const double inv_rho = 1.0 / cr[3];
+    // origin of the stored coordinates minus the centre
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = (ABS ? 0.0 : src_org[3 * t + k]) - cr[k];
+    for (int o = threadIdx.x; o < C * TT; o += blockDim.x) {
+        const int ch = o / TT;
+        const Mi m = mi_at(o % TT);
+        double acc = 0.0;
+        // padding records (index >= n_src) carry no charge
+        const int nj = (int)(n_src - (int64_t)t * G < G ? n_src - (int64_t)t * G : G);
+        for (int j = 0; j < nj; ++j) {
+            const T *q = rec + ((size_t)t * G + j) * RW;
+            double yp[3];
+            for (int k = 0; k < 3; ++k) yp[k] = (double)q[k] + d[k];
+            double pw = 1.0;
+            for (int e = 0; e < m.a; ++e) pw *= yp[0] * inv_rho;
+            for (int e = 0; e < m.b; ++e) pw *= yp[1] * inv_rho;
+            for (int e = 0; e < m.c; ++e) pw *= yp[2] * inv_rho;
+            acc += pw * charge<KID, T>(ch, q, yp);
+        }
+        moments[(size_t)t * C * TT + o] = (T)acc;
+    }
+}
+
+__device__ __forceinline__ double binom(int n, int k) {
+    double r = 1.0;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+}
+
+// Parent moments from children (one tree level).  Parents are nodes
+// [p_off, p_off + n_par); parent i has children c_off + 2i and, when it
+// exists (< c_off + n_child), c_off + 2i + 1.
+// M_p[a] = sum_child sum_{b <= a} C(a, b) (delta/rho_p)^(a-b) (rho_c/rho_p)^|b| M'_c[b]
+// with delta = c_child - c_parent and M'_c the child moments after the Psi
+// channels absorb the centre shift: Psi' = Psi + delta . q_src.
+template <int KID, typename T>
+__global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int n_par, int c_off,
+                                                       int n_child,
+                                                       const double *__restrict__ node_cr,
+                                                       T *__restrict__ moments) {
+    constexpr int C = TreeK<KID>::C;
+    const int i = blockIdx.x;
+    if (i >= n_par) return;
+    const int TT = n_terms(order);
+    const int pn = p_off + i;
+    const double *cp = node_cr + 4 * (size_t)pn;
+    for (int o = threadIdx.x; o < C * TT; o += blockDim.x) {
+        const int ch = o / TT;
+        const Mi m = mi_at(o % TT);
+        int psi = -1;
+        for (int s = 0; s < psi_count<KID>(); ++s)
+            if (psi_channel<KID>(s) == ch) psi = s;
+        double acc = 0.0;
+        for (int cc = 0; cc < 2; ++cc) {
+            const int ci = 2 * i + cc;
+            if (ci >= n_child) break;
+            const int cn = c_off + ci;
+            const double *ccr = node_cr + 4 * (size_t)cn;
+            const T *Mc = moments + (size_t)cn * C * TT;
+            double dl[3];
+            for (int k = 0; k < 3; ++k) dl[k] = (ccr[k] - cp[k]) / cp[3];
+            const double s = ccr[3] / cp[3];
+            double delta[3];
+            for (int k = 0; k < 3; ++k) delta[k] = ccr[k] - cp[k];
+            for (int ba = 0; ba <= m.a; ++ba)
+                for (int bb = 0; bb <= m.b; ++bb)
+                    for (int bc = 0; bc <= m.c; ++bc) {
+                        const int bi = mi_pos(ba, bb, bc);
+                        double coef = binom(m.a, ba) * binom(m.b, bb) * binom(m.c, bc);
+                        for (int e = 0; e < m.a - ba; ++e) coef *= dl[0];
+                        for (int e = 0; e < m.b - bb; ++e) coef *= dl[1];
+                        for (int e = 0; e < m.c - bc; ++e) coef *= dl[2];
+                        for (int e = 0; e < ba + bb + bc; ++e) coef *= s;
+                        double mv = (double)Mc[(size_t)ch * TT + bi];
+                        if (psi >= 0)
+                            for (int b = 0; b < 3; ++b)
+                                mv += delta[b] * (double)Mc[(size_t)psi_source<KID>(psi, b) * TT + bi];
+                        acc += coef * mv;
+                    }
+        }
+        moments[(size_t)pn * C * TT + o] = (T)acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One multipole node evaluated at one observation point.  a: derivatives up
+// to order P + 1 at u = (c - x)/rho; Rt = c - x (physical); M: the node's
+// scaled moments [C][n_terms(P)] (wave-uniform); out accumulates NO sums.
+template <int KID, int P, typename T>
+__device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__restrict__ M,
+                                         T inv_rho, const T mn[3], const KConst<T> &k, T *out) {
+    constexpr int TT = n_terms(P);
+    constexpr int C = TreeK<KID>::C;
+    (void)C;
+    // t_k(I) = (a_k + 1) a_{I + e_k}: the x_k-gradient weight of term I
+    auto tk = [&](auto ic, int kk) -> T {
+        constexpr int I = decltype(ic)::value;
+        constexpr Mi m = mi_at(I);
+        if (kk == 0) return T(m.a + 1) * a[mi_pos(m.a + 1, m.b, m.c)];
+        if (kk == 1) return T(m.b + 1) * a[mi_pos(m.a, m.b + 1, m.c)];
+        return T(m.c + 1) * a[mi_pos(m.a, m.b, m.c + 1)];
+    };
+    const T ir2 = inv_rho * inv_rho;
+    if constexpr (KID == TCT_U || KID == TCT_RA) {
+        T phi[3] = {0, 0, 0}, g[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+        static_for<0, TT>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            const T t0 = tk(ic, 0), t1 = tk(ic, 1), t2 = tk(ic, 2);
+            const T m0 = M[0 * TT + I], m1 = M[1 * TT + I], m2 = M[2 * TT + I];
+            phi[0] += m0 * a[I];
+            phi[1] += m1 * a[I];
+            phi[2] += m2 * a[I];
+            const T nI = M[3 * TT + I] + Rt[0] * m0 + Rt[1] * m1 + Rt[2] * m2;
+            g[0] += nI * t0;
+            g[1] += nI * t1;
+            g[2] += nI * t2;
+            if constexpr (KID == TCT_RA) {
+                const T mt = mn[0] * t0 + mn[1] * t1 + mn[2] * t2;
+                h[0] += m0 * mt;
+                h[1] += m1 * mt;
+                h[2] += m2 * mt;
+            }
+        });
+        if constexpr (KID == TCT_U) {
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) out[kk] += inv_rho * k.c0 * phi[kk] - ir2 * g[kk];
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                out[kk] += inv_rho * phi[kk];
+                out[3 + kk] -= ir2 * g[kk];
+                out[6 + kk] -= ir2 * h[kk];
+            }
+        }
+    } else if constexpr (KID == TCT_RT) {
+        T phi[3] = {0, 0, 0}, g[3] = {0, 0, 0}, h[3] = {0, 0, 0};
+        static_for<0, TT>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            const T t[3] = {tk(ic, 0), tk(ic, 1), tk(ic, 2)};
+            const T m0 = M[0 * TT + I], m1 = M[1 * TT + I], m2 = M[2 * TT + I];
+            phi[0] += m0 * a[I];
+            phi[1] += m1 * a[I];
+            phi[2] += m2 * a[I];
+            const T nI = M[3 * TT + I] + Rt[0] * m0 + Rt[1] * m1 + Rt[2] * m2;
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) g[kk] += nI * t[kk];
+#pragma unroll
+            for (int aa = 0; aa < 3; ++aa)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) h[aa] += M[(4 + 3 * aa + j) * TT + I] * t[j];
+        });
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+            out[kk] += inv_rho * phi[kk] - ir2 * (k.c0 * g[kk] + h[kk]);
+    } else {
+        // RH: M_km += A_km(Phi) - 2 (d_k Psi_m - x'_j d_k Phi_jm),
+        // A = CsRH1 tr(Phi) I + CsRH2 Phi^T - 2 Phi (k.c0 = CsRH1, k.c1 = CsRH2).
+        T phi[9], b[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) phi[q] = b[q] = T(0);
+        static_for<0, TT>([&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            const T t[3] = {tk(ic, 0), tk(ic, 1), tk(ic, 2)};
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm) {
+                const T g0 = M[(0 + mm) * TT + I], g1 = M[(3 + mm) * TT + I],
+                        g2 = M[(6 + mm) * TT + I];
+                phi[0 + mm] += g0 * a[I];
+                phi[3 + mm] += g1 * a[I];
+                phi[6 + mm] += g2 * a[I];
+                const T nI = M[(9 + mm) * TT + I] + Rt[0] * g0 + Rt[1] * g1 + Rt[2] * g2;
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) b[kk * 3 + mm] += nI * t[kk];
+            }
+        });
+        const T tr = phi[0] + phi[4] + phi[8];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm) {
+                T A = k.c1 * phi[mm * 3 + kk] - T(2) * phi[kk * 3 + mm];
+                if (kk == mm) A += k.c0 * tr;
+                out[kk * 3 + mm] += inv_rho * A + T(2) * ir2 * b[kk * 3 + mm];
+            }
+    }
+}
+
+// Evaluation: one wave (block of 64) per observation tile, grid.y chunks.
+// FP32 (ABS = false): obs = loc32 [6][n_obs_pad] local to obs_org per tile,
+// records local to src_org per tile.  FP64 (ABS = true): obs = soa64
+// absolute, records absolute.
+template <int KID, int P, typename T, bool ABS>
+__global__ __launch_bounds__(64) void tree_eval_kernel(
+    int n_obs_pad, const T *__restrict__ obs, const double *__restrict__ obs_org,
+    const T *__restrict__ rec, const double *__restrict__ src_org,
+    const double *__restrict__ node_cr, const T *__restrict__ moments,
+    const int *__restrict__ list_start, const int *__restrict__ list, KConst<T> k,
+    double *__restrict__ partial) {
+    constexpr int RW = rec_words<KID>();
+    constexpr int NO = n_out<KID>();
+    constexpr int C = TreeK<KID>::C;
+    constexpr int TT = n_terms(P);
+    const int tile = blockIdx.x;
+    const int chunks = gridDim.y;
+    const int cidx = blockIdx.y;
+    const int i = tile * G + (int)threadIdx.x;
+
+    T x[3], mn[3];
+    double oo[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = obs[(size_t)d * n_obs_pad + i];
+        mn[d] = (needs_obs_normal<KID>() || KID == TCT_RA) ? obs[(size_t)(3 + d) * n_obs_pad + i]
+                                                          : T(0);
+        oo[d] = ABS ? 0.0 : obs_org[3 * tile + d];
+    }
+    double acc64[NO];
+#pragma unroll
+    for (int q = 0; q < NO; ++q) acc64[q] = 0.0;
+
+    const int e0 = list_start[tile], e1 = list_start[tile + 1];
+    const int len = e1 - e0;
+    const int b0 = e0 + (int)(((long)len * cidx) / chunks);
+    const int b1 = e0 + (int)(((long)len * (cidx + 1)) / chunks);
+    for (int e = b0; e < b1; ++e) {
+        const int code = list[e];
+        T acc[NO];
+#pragma unroll
+        for (int q = 0; q < NO; ++q) acc[q] = T(0);
+        if (code < 0) {
+            const int t = -code - 1;
+            T xs[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                xs[d] = ABS ? x[d] : x[d] - (T)(src_org[3 * t + d] - oo[d]);
+            const T *r = rec + (size_t)t * G * RW;
+#pragma unroll 2
+            for (int j = 0; j < G; ++j) pair_eval<KID, T>(xs, mn, r + j * RW, k, acc);
+        } else {
+            const double *cr = node_cr + 4 * (size_t)code;
+            T Rt[3], u[3];
+            const T inv_rho = (T)(1.0 / cr[3]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                Rt[d] = (T)(cr[d] - oo[d]) - x[d];
+                u[d] = Rt[d] * inv_rho;
+            }
+            T a[n_terms(P + 1)];
+            taylor_derivs<P + 1, T>(u, a);
+            m2p_eval<KID, P, T>(a, Rt, moments + (size_t)code * C * TT, inv_rho, mn, k, acc);
+        }
+#pragma unroll
+        for (int q = 0; q < NO; ++q) acc64[q] += (double)acc[q];
+    }
+    double *out = partial + (size_t)cidx * NO * n_obs_pad;
+#pragma unroll
+    for (int q = 0; q < NO; ++q) out[(size_t)q * n_obs_pad + i] = acc64[q];
+}
+
+
+template <int KID, int P>
+inline void launch_eval(int prec, dim3 grid, hipStream_t st, int n_obs_pad, const void *obs,
+                        const double *obs_org, const void *rec, const double *src_org,
+                        const double *node_cr, const void *moments, const int *list_start,
+                        const int *list, const double *params, double eps2, double *partial) {
+    if (prec == TCT_F32)
+        hipLaunchKernelGGL((tree_eval_kernel<KID, P, float, false>), grid, dim3(G), 0, st,
+                           n_obs_pad, (const float *)obs, obs_org, (const float *)rec, src_org,
+                           node_cr, (const float *)moments, list_start, list,
+                           kernel_consts<float>(KID, params, eps2), partial);
+    else
+        hipLaunchKernelGGL((tree_eval_kernel<KID, P, double, true>), grid, dim3(G), 0, st,
+                           n_obs_pad, (const double *)obs, obs_org, (const double *)rec, src_org,
+                           node_cr, (const double *)moments, list_start, list,
+                           kernel_consts<double>(KID, params, 0.0), partial);
+}
+
+template <int KID>
+inline int eval_order(int order, int prec, dim3 grid, hipStream_t st, int n_obs_pad,
+                      const void *obs, const double *obs_org, const void *rec,
+                      const double *src_org, const double *node_cr, const void *moments,
+                      const int *list_start, const int *list, const double *params, double eps2,
+                      double *partial) {
+#define TCT_EV(P)                                                                              \
+    launch_eval<KID, P>(prec, grid, st, n_obs_pad, obs, obs_org, rec, src_org, node_cr,        \
+                        moments, list_start, list, params, eps2, partial)
+    switch (order) {
+    case 2: TCT_EV(2); break;
+    case 3: TCT_EV(3); break;
+    case 4: TCT_EV(4); break;
+    case 5: TCT_EV(5); break;
+    case 6: TCT_EV(6); break;
+    default:
+        return set_error(TCT_EUNSUPPORTED, "tree_eval: order %d not built (2-6)", order);
+    }
+#undef TCT_EV
+    return TCT_OK;
+}
+
+
+// Evaluation entry of one kernel (defined in far_tree_eval_<k>.hip).
+template <int KID>
+int tree_eval_kernel_entry(int order, int prec, dim3 grid, hipStream_t st, int n_obs_pad,
+                           const void *obs, const double *obs_org, const void *rec,
+                           const double *src_org, const double *node_cr,
+                           const void *moments, const int *list_start, const int *list,
+                           const double *params, double eps2, double *partial);
+template <>
+int tree_eval_kernel_entry<TCT_U>(int order, int prec, dim3 grid, hipStream_t st,
+                                int n_obs_pad, const void *obs, const double *obs_org,
+                                const void *rec, const double *src_org,
+                                const double *node_cr, const void *moments,
+                                const int *list_start, const int *list,
+                                const double *params, double eps2, double *partial);
+template <>
+int tree_eval_kernel_entry<TCT_RT>(int order, int prec, dim3 grid, hipStream_t st,
+                                int n_obs_pad, const void *obs, const double *obs_org,
+                                const void *rec, const double *src_org,
+                                const double *node_cr, const void *moments,
+                                const int *list_start, const int *list,
+                                const double *params, double eps2, double *partial);
+template <>
+int tree_eval_kernel_entry<TCT_RA>(int order, int prec, dim3 grid, hipStream_t st,
+                                int n_obs_pad, const void *obs, const double *obs_org,
+                                const void *rec, const double *src_org,
+                                const double *node_cr, const void *moments,
+                                const int *list_start, const int *list,
+                                const double *params, double eps2, double *partial);
+template <>
+int tree_eval_kernel_entry<TCT_RH>(int order, int prec, dim3 grid, hipStream_t st,
+                                int n_obs_pad, const void *obs, const double *obs_org,
+                                const void *rec, const double *src_org,
+                                const double *node_cr, const void *moments,
+                                const int *list_start, const int *list,
+                                const double *params, double eps2, double *partial);
+
+}  // namespace tct

@@ -1,0 +1,104 @@
+// Treecode farfield: tile moments, upward pass and the C ABI.  The kernel
+// formulation and layout are described in far_tree.h.
+#include "far_tree.h"
+
+using namespace tct;
+
+static int tree_kernel_ok(int kernel) {
+    return kernel == TCT_U || kernel == TCT_RT || kernel == TCT_RA || kernel == TCT_RH;
+}
+
+extern "C" int tct_tree_p2m(int kernel, int prec, int order, int64_t n_src, int64_t n_tiles,
+                            const void *rec, const double *src_org, const double *node_cr,
+                            void *moments, void *stream) {
+    TCT_CHECK_ARG(tree_kernel_ok(kernel), "tree_p2m: kernel %d has no treecode form", kernel);
+    TCT_CHECK_ARG(prec == TCT_F32 || prec == TCT_F64, "tree_p2m: precision %d", prec);
+    TCT_CHECK_ARG(order >= 0 && order <= TCT_TREE_MAX_ORDER, "tree_p2m: order %d", order);
+    TCT_CHECK_ARG(n_tiles > 0 && n_tiles < (1LL << 31) && n_src > (n_tiles - 1) * G &&
+                      n_src <= n_tiles * G,
+                  "tree_p2m: n_src %lld does not fill n_tiles %lld", (long long)n_src,
+                  (long long)n_tiles);
+    TCT_CHECK_ARG(rec && node_cr && moments && (prec == TCT_F64 || src_org),
+                  "tree_p2m: null pointer");
+    hipStream_t st = as_stream(stream);
+    const dim3 grid((unsigned)n_tiles), block(256);
+#define TCT_P2M(KID)                                                                          \
+    if (prec == TCT_F32)                                                                      \
+        hipLaunchKernelGGL((tree_p2m_kernel<KID, float, false>), grid, block, 0, st, order,   \
+                           n_src, (const float *)rec, src_org, node_cr, (float *)moments);    \
+    else                                                                                      \
+        hipLaunchKernelGGL((tree_p2m_kernel<KID, double, true>), grid, block, 0, st, order,   \
+                           n_src, (const double *)rec, src_org, node_cr, (double *)moments)
+    switch (kernel) {
+    case TCT_U: TCT_P2M(TCT_U); break;
+    case TCT_RT: TCT_P2M(TCT_RT); break;
+    case TCT_RA: TCT_P2M(TCT_RA); break;
+    default: TCT_P2M(TCT_RH); break;
+    }
+#undef TCT_P2M
+    TCT_CHECK_LAUNCH("tree_p2m");
+    return TCT_OK;
+}
+
+extern "C" int tct_tree_m2m(int kernel, int prec, int order, int p_off, int n_par, int c_off,
+                            int n_child, const double *node_cr, void *moments, void *stream) {
+    TCT_CHECK_ARG(tree_kernel_ok(kernel), "tree_m2m: kernel %d has no treecode form", kernel);
+    TCT_CHECK_ARG(prec == TCT_F32 || prec == TCT_F64, "tree_m2m: precision %d", prec);
+    TCT_CHECK_ARG(order >= 0 && order <= TCT_TREE_MAX_ORDER, "tree_m2m: order %d", order);
+    TCT_CHECK_ARG(n_par > 0 && n_child > 0 && n_child <= 2 * n_par && n_child > 2 * (n_par - 1) &&
+                      p_off >= 0 && c_off >= 0 && c_off + n_child <= p_off,
+                  "tree_m2m: bad level p_off=%d n_par=%d c_off=%d n_child=%d", p_off, n_par,
+                  c_off, n_child);
+    TCT_CHECK_ARG(node_cr && moments, "tree_m2m: null pointer");
+    hipStream_t st = as_stream(stream);
+    const dim3 grid((unsigned)n_par), block(256);
+#define TCT_M2M(KID)                                                                           \
+    if (prec == TCT_F32)                                                                       \
+        hipLaunchKernelGGL((tree_m2m_kernel<KID, float>), grid, block, 0, st, order, p_off,    \
+                           n_par, c_off, n_child, node_cr, (float *)moments);                  \
+    else                                                                                       \
+        hipLaunchKernelGGL((tree_m2m_kernel<KID, double>), grid, block, 0, st, order, p_off,   \
+                           n_par, c_off, n_child, node_cr, (double *)moments)
+    switch (kernel) {
+    case TCT_U: TCT_M2M(TCT_U); break;
+    case TCT_RT: TCT_M2M(TCT_RT); break;
+    case TCT_RA: TCT_M2M(TCT_RA); break;
+    default: TCT_M2M(TCT_RH); break;
+    }
+#undef TCT_M2M
+    TCT_CHECK_LAUNCH("tree_m2m");
+    return TCT_OK;
+}
+
+extern "C" int tct_tree_eval(int kernel, int prec, int order, int n_obs_tiles, int n_obs_pad,
+                             const void *obs, const double *obs_org, const void *rec,
+                             const double *src_org, const double *node_cr, const void *moments,
+                             const int *list_start, const int *list, const double *params_host,
+                             double eps2, int chunks, double *out_partial, void *stream) {
+    TCT_CHECK_ARG(tree_kernel_ok(kernel), "tree_eval: kernel %d has no treecode form", kernel);
+    TCT_CHECK_ARG(prec == TCT_F32 || prec == TCT_F64, "tree_eval: precision %d", prec);
+    TCT_CHECK_ARG(n_obs_tiles > 0 && n_obs_pad == n_obs_tiles * G,
+                  "tree_eval: n_obs_pad %d must be 64 * n_obs_tiles (%d)", n_obs_pad,
+                  n_obs_tiles);
+    TCT_CHECK_ARG(chunks >= 1 && chunks <= 65535, "tree_eval: chunks %d", chunks);
+    TCT_CHECK_ARG(obs && rec && node_cr && moments && list_start && list && params_host &&
+                      out_partial && (prec == TCT_F64 || (obs_org && src_org)),
+                  "tree_eval: null pointer");
+    hipStream_t st = as_stream(stream);
+    const dim3 grid((unsigned)n_obs_tiles, (unsigned)chunks);
+    int rc;
+#define TCT_EK(KID)                                                                           \
+    rc = tree_eval_kernel_entry<KID>(order, prec, grid, st, n_obs_pad, obs, obs_org, rec,     \
+                                     src_org, node_cr, moments, list_start, list, params_host, \
+                                     eps2, out_partial)
+    switch (kernel) {
+    case TCT_U: TCT_EK(TCT_U); break;
+    case TCT_RT: TCT_EK(TCT_RT); break;
+    case TCT_RA: TCT_EK(TCT_RA); break;
+    default: TCT_EK(TCT_RH); break;
+    }
+#undef TCT_EK
+    if (rc != TCT_OK) return rc;
+    TCT_CHECK_LAUNCH("tree_eval");
+    return TCT_OK;
+}

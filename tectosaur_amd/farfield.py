@@ -170,12 +170,15 @@ class FarfieldEngine:
                N.ptr(tri_of_pt), N.ptr(qphi), N.ptr(wj), N.ptr(tri_curl), N.ptr(v_dev),
                self.params.ctypes.data, N.ptr(self.rec(prec)), N.stream_ptr())
 
-    def scatter_curl(self, prec, n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl, out_dev):
-        cfg = self.tile_cfg[prec]
+    def scatter_curl(self, prec, n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl, out_dev,
+                     partial=None, chunks=None):
+        """``partial``/``chunks`` override the N-body's own buffer (the
+        treecode writes a buffer of its own with its own chunk count)."""
+        chunks = chunks or self.tile_cfg[prec].src_chunks
+        partial = partial if partial is not None else self.partial(prec)
         N.call('tct_far_scatter_curl', self.kid, n_tris, nq, N.ptr(pt_of_tri), N.ptr(qphi),
-               N.ptr(wj_obs), N.ptr(tri_curl), self.n_obs_pad, cfg.src_chunks,
-               N.ptr(self.partial(prec)), self.params.ctypes.data, N.ptr(out_dev),
-               N.stream_ptr())
+               N.ptr(wj_obs), N.ptr(tri_curl), self.n_obs_pad, chunks, N.ptr(partial),
+               self.params.ctypes.data, N.ptr(out_dev), N.stream_ptr())
 
     def run(self, prec, cfg=None):
         """Launch the N-body kernel; returns the partial-sum buffer."""
@@ -189,16 +192,18 @@ class FarfieldEngine:
                float(self.eps2), ctypes_ref(cfg), N.ptr(out), N.stream_ptr())
         return out
 
-    def reduce_pts(self, prec, out_dev):
-        cfg = self.tile_cfg[prec]
-        N.call('tct_far_reduce_pts', self.n_obs, self.n_obs_pad, cfg.src_chunks,
-               N.ptr(self.partial(prec)), N.ptr(out_dev), N.stream_ptr())
+    def reduce_pts(self, prec, out_dev, partial=None, chunks=None):
+        chunks = chunks or self.tile_cfg[prec].src_chunks
+        partial = partial if partial is not None else self.partial(prec)
+        N.call('tct_far_reduce_pts', self.n_obs, self.n_obs_pad, chunks, N.ptr(partial),
+               N.ptr(out_dev), N.stream_ptr())
 
-    def scatter_tris(self, prec, n_tris, nq, pt_of_tri, qphi, wj_obs, out_dev):
-        cfg = self.tile_cfg[prec]
+    def scatter_tris(self, prec, n_tris, nq, pt_of_tri, qphi, wj_obs, out_dev, partial=None,
+                     chunks=None):
+        chunks = chunks or self.tile_cfg[prec].src_chunks
+        partial = partial if partial is not None else self.partial(prec)
         N.call('tct_far_scatter_tris', n_tris, nq, N.ptr(pt_of_tri), N.ptr(qphi), N.ptr(wj_obs),
-               self.n_obs_pad, cfg.src_chunks, N.ptr(self.partial(prec)), N.ptr(out_dev),
-               N.stream_ptr())
+               self.n_obs_pad, chunks, N.ptr(partial), N.ptr(out_dev), N.stream_ptr())
 
     @property
     def pair_evals(self):

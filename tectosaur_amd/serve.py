@@ -30,7 +30,7 @@ import numpy as np
 import torch
 
 from .solver import ConstrainedOperator, make_state
-from .traction_slip import (TractionSlipProblem, fault_mesh, prepare_problem,
+from .traction_slip import (DEFAULT_TREE, TractionSlipProblem, fault_mesh, prepare_problem,
                             request_traction)
 
 
@@ -85,17 +85,19 @@ class _Active:
         self.nbytes = nbytes
 
 
-def _prepare(req, orders):
+def _prepare(req, orders, tree=None):
     t0 = time.perf_counter()
     pts, tris = fault_mesh(req)
     t = request_traction(req, tris.shape[0])
-    prep = prepare_problem(pts, tris, t, orders=orders)
+    prep = prepare_problem(pts, tris, t, orders=orders, tree=tree)
     prep['prep_s'] = time.perf_counter() - t0
     return prep
 
 
 def _build(prep, mode, tol, restart, maxiter):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
+    if mode == 'tree' and prep.get('tree') is None:
+        raise ValueError("mode 'tree' needs requests prepared with a treecode")
     prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
     b = op.cmT.dot_dev(prob.rhs)
@@ -117,7 +119,7 @@ def _finalize(a, now, keep_slip):
 
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
-                 hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4):
+                 hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None):
         self.mode = mode
         self.max_active = max_active
         self.tol = tol
@@ -125,6 +127,8 @@ class Server:
         self.maxiter = maxiter
         self.hbm_fraction = hbm_fraction
         self.orders = orders
+        # mode 'tree': treecode parameters of the inner farfield
+        self.tree = (tree or dict(DEFAULT_TREE)) if mode == 'tree' else None
         self.keep_slip = keep_slip
         self.prep_workers = prep_workers
         self.ticks = 0
@@ -155,7 +159,8 @@ class Server:
                 while pool is not None and si < len(pending) and si < qi + max_ahead:
                     if realtime and pending[si].arrival > now:
                         break
-                    futures[pending[si].rid] = pool.submit(_prepare, pending[si], self.orders)
+                    futures[pending[si].rid] = pool.submit(_prepare, pending[si], self.orders,
+                                                           self.tree)
                     si += 1
                 # admission (FIFO) of prepared requests
                 while qi < len(pending) and len(active) < self.max_active:
@@ -178,7 +183,7 @@ class Server:
                         rec.arrival = 0.0
                     rec.admitted = clock()
                     if prep is None:
-                        prep = _prepare(req, self.orders)
+                        prep = _prepare(req, self.orders, self.tree)
                     rec.prep_s = prep['prep_s']
                     prob, op, state = _build(prep, self.mode, self.tol, self.restart,
                                              self.maxiter)

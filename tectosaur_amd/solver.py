@@ -42,7 +42,7 @@ class ConstrainedOperator:
         self.shape = (cm.shape[1], cm.shape[1])
         self._u = torch.empty(cm.shape[0], dtype=torch.float64, device=self.device)
         self._y = torch.empty(cm.shape[0], dtype=torch.float64, device=self.device)
-        self.n_matvec = {'f32': 0, 'f64': 0}
+        self.n_matvec = {'f32': 0, 'f64': 0, 'tree': 0}
 
     def apply(self, x, out=None, prec='f64'):
         if out is None:
@@ -241,6 +241,9 @@ MODES = {
     # name: (inner_prec, outer_prec, inner_reduction)
     'f64': ('f64', 'f64', None),
     'mixed': ('f32', 'f64', 1e-4),
+    # FP32 treecode farfield in the Arnoldi cycles (operators with a tree
+    # attached), exact FP64 direct farfield for every residual
+    'tree': ('tree', 'f64', 1e-4),
 }
 
 
@@ -256,7 +259,9 @@ def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxi
 
     mode 'f64': the plain FP64 path (FP64 farfield everywhere).
     mode 'mixed': FP32 farfield in the Arnoldi cycles, FP64 residual at
-    every restart (GMRES-based iterative refinement)."""
+    every restart (GMRES-based iterative refinement).
+    mode 'tree': as 'mixed' with the FP32 treecode farfield in the cycles
+    (A's farfield must have a tree attached); residuals stay FP64 direct."""
     if mode not in MODES:
         raise ValueError('mode must be one of %s' % sorted(MODES))
     op = ConstrainedOperator(A, cm)

@@ -114,29 +114,64 @@ class TriToTriDirectFarfieldOp:
         if self.curl:
             self.curl_src = torch.as_tensor(plan['curl_src'], **f64)
             self.curl_obs = torch.as_tensor(plan['curl_obs'], **f64)
+        self._plan_pts = (plan['o_pts'], plan['s_pts'])
+        self.tree = None
 
-    @property
-    def pair_evals(self):
-        return self.engine.pair_evals
+    def attach_tree(self, order=4, mac=3.0, pts_per_cell=64, chunks=None, plan=None):
+        """Add a treecode evaluation of this operator (kernels U3, RT3, RA3,
+        RH3), used by ``dot_dev(..., prec='tree')`` (FP32) and
+        ``tree_dot_dev``; the direct FP32/FP64 paths stay available.
+        ``plan``: a precomputed ``fmm.tree_plan`` for these point sets."""
+        from .fmm import TreeFarfield
+        self.tree = TreeFarfield(self.engine, self._plan_pts[0], self._plan_pts[1], order=order,
+                                 mac=mac, pts_per_cell=pts_per_cell, chunks=chunks, plan=plan)
+        return self.tree
 
-    def dot_dev(self, v_dev, out_dev=None, prec=None):
-        """Device matvec: v_dev float64 [n_src*9] -> float64 [n_obs*9]."""
-        prec = prec or self.prec
+    def tree_dot_dev(self, v_dev, out_dev=None, prec='f32'):
+        """Matvec with the treecode in place of the direct N-body."""
+        if self.tree is None:
+            raise N.TctError('no treecode attached (attach_tree)')
         if out_dev is None:
             out_dev = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+        e = self.engine
+        self._gather(v_dev, prec)
+        part = self.tree.run(prec)
+        self._scatter(prec, out_dev, part, self.tree.chunks)
+        return out_dev
+
+    def _gather(self, v_dev, prec):
         e = self.engine
         if self.curl:
             e.gather_curl(v_dev, self.tri_of_pt, self.qphi, self.wj_src, self.curl_src, self.nq,
                           prec)
         else:
             e.gather_tris(v_dev, self.tri_of_pt, self.qphi, self.wj_src, self.nq, prec)
-        e.run(prec)
+
+    def _scatter(self, prec, out_dev, partial=None, chunks=None):
+        e = self.engine
         if e.n_out == 9:
             e.scatter_curl(prec, self.n_obs, self.nq, self.pt_of_tri, self.qphi, self.wj_obs,
-                           self.curl_obs, out_dev)
+                           self.curl_obs, out_dev, partial, chunks)
         else:
             e.scatter_tris(prec, self.n_obs, self.nq, self.pt_of_tri, self.qphi, self.wj_obs,
-                           out_dev)
+                           out_dev, partial, chunks)
+
+    @property
+    def pair_evals(self):
+        return self.engine.pair_evals
+
+    def dot_dev(self, v_dev, out_dev=None, prec=None):
+        """Device matvec: v_dev float64 [n_src*9] -> float64 [n_obs*9].
+        prec: 'f32' or 'f64' (direct N-body), or 'tree' (FP32 treecode, after
+        attach_tree)."""
+        prec = prec or self.prec
+        if prec == 'tree':
+            return self.tree_dot_dev(v_dev, out_dev, 'f32')
+        if out_dev is None:
+            out_dev = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
+        self._gather(v_dev, prec)
+        self.engine.run(prec)
+        self._scatter(prec, out_dev)
         return out_dev
 
     def dot(self, v):

@@ -118,11 +118,15 @@ def fault_mesh(req):
     return mesh_mod.make_rect(nx + 1, nz + 1, corners)
 
 
-def prepare_problem(pts, tris, traction, sm=3e10, orders=None, K_far='elasticRH3'):
+DEFAULT_TREE = dict(order=4, mac=2.5)
+
+
+def prepare_problem(pts, tris, traction, sm=3e10, orders=None, K_far='elasticRH3', tree=None):
     """The host-only half of a traction -> slip set-up (mesh-side work that
     needs no device): pair classification, farfield layout, mass matrix,
-    constraint matrix and right-hand side.  Safe to run on a worker thread
-    while the device is busy with other solves."""
+    constraint matrix and right-hand side, and (``tree`` = dict(order,
+    mac)) the treecode's tree and interaction lists.  Safe to run on a
+    worker thread while the device is busy with other solves."""
     from .nearfield_op import nearfield_plan
     from .sparse_farfield_op import farfield_plan
     from .kernels import kernels as KERNELS
@@ -134,11 +138,15 @@ def prepare_problem(pts, tris, traction, sm=3e10, orders=None, K_far='elasticRH3
     K = KERNELS[K_far]
     M = MassOp(o['quad_mass_order'], pts, tris)
     t = np.asarray(traction, dtype=np.float64).reshape(-1)
-    return dict(pts=pts, tris=tris, orders=o, K_far=K_far,
-                near_plan=nearfield_plan(pts, tris, idx, idx, o['near_threshold']),
-                far_plan=farfield_plan(o['nq_far'], pts, tris, idx, idx,
-                                       K.surf_curl_obs or K.surf_curl_src),
-                M=M, cm=continuity_cm(tris), rhs=-M.dot(t / sm))
+    far_plan = farfield_plan(o['nq_far'], pts, tris, idx, idx, K.surf_curl_obs or K.surf_curl_src)
+    out = dict(pts=pts, tris=tris, orders=o, K_far=K_far,
+               near_plan=nearfield_plan(pts, tris, idx, idx, o['near_threshold']),
+               far_plan=far_plan, M=M, cm=continuity_cm(tris), rhs=-M.dot(t / sm), tree=None)
+    if tree is not None:
+        from .fmm import tree_plan
+        out['tree'] = dict(tree, plan=tree_plan(far_plan['o_pts'], far_plan['s_pts'],
+                                                tree['mac']))
+    return out
 
 
 class TractionSlipProblem:
@@ -160,6 +168,9 @@ class TractionSlipProblem:
             near_plan=prepared['near_plan'], far_plan=prepared['far_plan'],
             keep_uncorrected=False)
         self.device = self.H.device
+        if prepared.get('tree') is not None:
+            tr = prepared['tree']
+            self.H.farfield.attach_tree(order=tr['order'], mac=tr['mac'], plan=tr['plan'])
         self.M = prepared['M']
         self.cm = prepared['cm']
         self.rhs = torch.as_tensor(prepared['rhs'], dtype=torch.float64, device=self.device)

@@ -56,6 +56,25 @@ def test_solve_residual_under_oracle_operator(oracle, mode, K_far):
         assert res.n_matvec['f32'] > res.n_matvec['f64']
 
 
+@pytest.mark.parametrize('order,mac', [(3, 2.0), (4, 2.5)])
+def test_tree_solve_residual_under_oracle_operator(oracle, order, mac):
+    # FP32 treecode farfield in the Arnoldi cycles, exact FP64 residuals:
+    # the converged residual is the exact operator's, checked with the oracle.
+    from tectosaur_amd.solver import solve_constrained
+    from tectosaur_amd.traction_slip import prepare_problem
+    pts, tris, t = small_problem(600, 9)
+    prep = prepare_problem(pts, tris, t, orders=ORD, tree=dict(order=order, mac=mac))
+    prob = TractionSlipProblem(None, None, None, prepared=prep)
+    assert prob.H.farfield.tree.stats['m2p'] > 0
+    u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=1e-8, mode='tree', restart=100)
+    assert res.converged and res.rel_residual <= 1e-8
+    assert res.n_matvec['tree'] > res.n_matvec['f64'] > 0 and res.n_matvec['f32'] == 0
+    u = u.cpu().numpy()
+    b = prob.cm.T @ prob.rhs.cpu().numpy()
+    r = b - prob.cm.T @ oracle_apply(oracle, pts, tris, u)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.1e-8
+
+
 def test_mixed_matches_f64_solution():
     from tectosaur_amd.solver import solve_constrained
     pts, tris, t = small_problem(300, 7)
@@ -80,6 +99,20 @@ def test_server_result_same_alone_and_in_mix():
         assert rm.status == ra.status == 'converged'
         assert rm.iterations == ra.iterations
         # identical kernels on identical buffers: agreement to roundoff (bitwise in practice)
+        assert np.abs(rm.slip - ra.slip).max() <= 1e-12 * np.abs(ra.slip).max()
+
+
+def test_tree_server_result_same_alone_and_in_mix():
+    from tectosaur_amd.serve import Server
+    from tectosaur_amd.traction_slip import random_requests
+    reqs = random_requests(4, rate=0.0, seed=8, n_min=400, n_max=1500)
+    mix = Server(mode='tree', max_active=4, restart=100, orders=ORD, keep_slip=True)
+    recs_mix = mix.run(reqs, realtime=False)
+    for req, rm in zip(reqs, recs_mix):
+        ra = Server(mode='tree', max_active=1, restart=100, orders=ORD,
+                    keep_slip=True).run([req], realtime=False)[0]
+        assert rm.status == ra.status == 'converged' and rm.rel_residual <= 1e-8
+        assert rm.iterations == ra.iterations
         assert np.abs(rm.slip - ra.slip).max() <= 1e-12 * np.abs(ra.slip).max()
 
 
