@@ -188,17 +188,22 @@ int tct_tree_p2m(int kernel, int prec, int order, int64_t n_src, int64_t n_tiles
 int tct_tree_m2m(int kernel, int prec, int order, int p_off, int n_par, int c_off, int n_child,
                  const double *node_cr, void *moments, void *stream);
 
-/* Evaluation over per-observation-tile interaction lists: list_start
- * [n_obs_tiles + 1] int32, list int32 entries e >= 0 (m2p from node e) or
- * e < 0 (p2p with source tile -(e + 1)).  obs = loc32 (FP32, with obs_org)
- * or soa64 (FP64); rec/src_org as for tct_far_p2p.  Each list is split into
- * `chunks` parts; out_partial [chunks][3 or 9][n_obs_pad] as tct_far_p2p,
- * so tct_far_reduce_pts / tct_far_scatter_tris / tct_far_scatter_curl apply. */
-int tct_tree_eval(int kernel, int prec, int order, int n_obs_tiles, int n_obs_pad,
-                  const void *obs, const double *obs_org, const void *rec, const double *src_org,
-                  const double *node_cr, const void *moments, const int *list_start,
-                  const int *list, const double *params_host, double eps2, int chunks,
-                  double *out_partial, void *stream);
+/* Evaluation over interaction-list segments: list int32 entries e >= 0
+ * (m2p from node e) or e < 0 (p2p with source tile -(e + 1)); seg
+ * [n_seg][3] int32 = (observation tile, first entry, end entry), one
+ * wavefront each.  obs = loc32 (FP32, with obs_org) or soa64 (FP64);
+ * rec/src_org as for tct_far_p2p.  seg_partial [n_seg][3 or 9][64] f64. */
+int tct_tree_eval(int kernel, int prec, int order, int n_obs_pad, const void *obs,
+                  const double *obs_org, const void *rec, const double *src_org,
+                  const double *node_cr, const void *moments, int n_seg, const int *seg,
+                  const int *list, const double *params_host, double eps2, double *seg_partial,
+                  void *stream);
+
+/* out_partial [3 or 9][n_obs_pad] = sum of each observation tile's segments
+ * [seg_off[t], seg_off[t + 1]) in order (the layout of tct_far_p2p with one
+ * chunk, for tct_far_reduce_pts / scatter_tris / scatter_curl). */
+int tct_tree_reduce(int kernel, int n_obs_tiles, int n_obs_pad, const int *seg_off,
+                    const double *seg_partial, double *out_partial, void *stream);
 
 /* ---------------- nearfield pair integrator (FP64) ----------------------
  * Reference: single_pairs{Z,N} / single_pairsN_adj (tectosaur/assemble.cl

@@ -54,8 +54,9 @@ SIGNATURES = {
     'tct_far_reduce_pts': [_i64, _i64, _int, _vp, _vp, _vp],
     'tct_tree_p2m': [_int, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     'tct_tree_m2m': [_int, _int, _int, _int, _int, _int, _int, _vp, _vp, _vp],
-    'tct_tree_eval': [_int, _int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                      _vp, _dbl, _int, _vp, _vp],
+    'tct_tree_eval': [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _vp,
+                      _vp, _dbl, _vp, _vp],
+    'tct_tree_reduce': [_int, _int, _int, _vp, _vp, _vp, _vp],
     'tct_far_scatter_tris': [_i64, _int, _vp, _vp, _vp, _i64, _int, _vp, _vp, _vp],
     'tct_pairs_integrate': [_int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _vp, _int,
                             _vp, _vp],

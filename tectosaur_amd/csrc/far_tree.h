@@ -190,62 +190,85 @@ __global__ __launch_bounds__(256) void tree_p2m_kernel(int order, int64_t n_src,
     }
 }
 
-__device__ __forceinline__ double binom(int n, int k) {
-    double r = 1.0;
-    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
-    return r;
-}
-
 // Parent moments from children (one tree level).  Parents are nodes
 // [p_off, p_off + n_par); parent i has children c_off + 2i and, when it
 // exists (< c_off + n_child), c_off + 2i + 1.
 // M_p[a] = sum_child sum_{b <= a} C(a, b) (delta/rho_p)^(a-b) (rho_c/rho_p)^|b| M'_c[b]
 // with delta = c_child - c_parent and M'_c the child moments after the Psi
-// channels absorb the centre shift: Psi' = Psi + delta . q_src.
+// channels absorb the centre shift: Psi' = Psi + delta . q_src.  One block
+// per parent: the children's shifted moments, the binomials and the powers
+// of delta/rho_p and rho_c/rho_p are staged in LDS first.
 template <int KID, typename T>
 __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int n_par, int c_off,
                                                        int n_child,
                                                        const double *__restrict__ node_cr,
                                                        T *__restrict__ moments) {
     constexpr int C = TreeK<KID>::C;
+    constexpr int PM = TCT_TREE_MAX_ORDER + 1;
+    constexpr int TMAX = n_terms(TCT_TREE_MAX_ORDER);
+    __shared__ double mc[2][C * TMAX];
+    __shared__ double binom_t[PM][PM];
+    __shared__ double pw_d[2][3][PM];
+    __shared__ double pw_s[2][PM];
+    __shared__ double delta[2][3];
     const int i = blockIdx.x;
     if (i >= n_par) return;
     const int TT = n_terms(order);
     const int pn = p_off + i;
     const double *cp = node_cr + 4 * (size_t)pn;
-    for (int o = threadIdx.x; o < C * TT; o += blockDim.x) {
+    const int nc = (2 * i + 1 < n_child) ? 2 : 1;
+    const int tid = threadIdx.x;
+    if (tid < PM) {
+        double v = 1.0;  // row tid of Pascal's triangle
+        for (int kk = 0; kk < PM; ++kk) {
+            binom_t[tid][kk] = kk <= tid ? v : 0.0;
+            v = v * (tid - kk) / (kk + 1);
+        }
+    } else if (tid >= 32 && tid < 32 + 2 * 4) {
+        const int cc = (tid - 32) / 4, k = (tid - 32) % 4;
+        if (cc < nc) {
+            const double *ccr = node_cr + 4 * (size_t)(c_off + 2 * i + cc);
+            const double base = k < 3 ? (ccr[k] - cp[k]) / cp[3] : ccr[3] / cp[3];
+            double v = 1.0;
+            for (int e = 0; e < PM; ++e) {
+                if (k < 3)
+                    pw_d[cc][k][e] = v;
+                else
+                    pw_s[cc][e] = v;
+                v *= base;
+            }
+            if (k < 3) delta[cc][k] = ccr[k] - cp[k];
+        }
+    }
+    for (int o = tid; o < nc * C * TT; o += blockDim.x) {
+        const int cc = o / (C * TT), r = o % (C * TT);
+        mc[cc][r] = (double)moments[(size_t)(c_off + 2 * i + cc) * C * TT + r];
+    }
+    __syncthreads();
+    // Psi channels absorb the shift of centre (read the unshifted sources)
+    for (int o = tid; o < nc * psi_count<KID>() * TT; o += blockDim.x) {
+        const int cc = o / (psi_count<KID>() * TT), r = o % (psi_count<KID>() * TT);
+        const int ps = r / TT, bi = r % TT;
+        double add = 0.0;
+        for (int b = 0; b < 3; ++b) add += delta[cc][b] * mc[cc][psi_source<KID>(ps, b) * TT + bi];
+        mc[cc][psi_channel<KID>(ps) * TT + bi] += add;
+    }
+    __syncthreads();
+    for (int o = tid; o < C * TT; o += blockDim.x) {
         const int ch = o / TT;
         const Mi m = mi_at(o % TT);
-        int psi = -1;
-        for (int s = 0; s < psi_count<KID>(); ++s)
-            if (psi_channel<KID>(s) == ch) psi = s;
         double acc = 0.0;
-        for (int cc = 0; cc < 2; ++cc) {
-            const int ci = 2 * i + cc;
-            if (ci >= n_child) break;
-            const int cn = c_off + ci;
-            const double *ccr = node_cr + 4 * (size_t)cn;
-            const T *Mc = moments + (size_t)cn * C * TT;
-            double dl[3];
-            for (int k = 0; k < 3; ++k) dl[k] = (ccr[k] - cp[k]) / cp[3];
-            const double s = ccr[3] / cp[3];
-            double delta[3];
-            for (int k = 0; k < 3; ++k) delta[k] = ccr[k] - cp[k];
-            for (int ba = 0; ba <= m.a; ++ba)
-                for (int bb = 0; bb <= m.b; ++bb)
-                    for (int bc = 0; bc <= m.c; ++bc) {
-                        const int bi = mi_pos(ba, bb, bc);
-                        double coef = binom(m.a, ba) * binom(m.b, bb) * binom(m.c, bc);
-                        for (int e = 0; e < m.a - ba; ++e) coef *= dl[0];
-                        for (int e = 0; e < m.b - bb; ++e) coef *= dl[1];
-                        for (int e = 0; e < m.c - bc; ++e) coef *= dl[2];
-                        for (int e = 0; e < ba + bb + bc; ++e) coef *= s;
-                        double mv = (double)Mc[(size_t)ch * TT + bi];
-                        if (psi >= 0)
-                            for (int b = 0; b < 3; ++b)
-                                mv += delta[b] * (double)Mc[(size_t)psi_source<KID>(psi, b) * TT + bi];
-                        acc += coef * mv;
-                    }
+        for (int cc = 0; cc < nc; ++cc) {
+            const double *M = mc[cc] + ch * TT;
+            for (int ba = 0; ba <= m.a; ++ba) {
+                const double ca = binom_t[m.a][ba] * pw_d[cc][0][m.a - ba];
+                for (int bb = 0; bb <= m.b; ++bb) {
+                    const double cb = ca * binom_t[m.b][bb] * pw_d[cc][1][m.b - bb];
+                    for (int bc = 0; bc <= m.c; ++bc)
+                        acc += cb * binom_t[m.c][bc] * pw_d[cc][2][m.c - bc] *
+                               pw_s[cc][ba + bb + bc] * M[mi_pos(ba, bb, bc)];
+                }
+            }
         }
         moments[(size_t)pn * C * TT + o] = (T)acc;
     }
@@ -255,12 +278,22 @@ __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int
 // One multipole node evaluated at one observation point.  a: derivatives up
 // to order P + 1 at u = (c - x)/rho; Rt = c - x (physical); M: the node's
 // scaled moments [C][n_terms(P)] (wave-uniform); out accumulates NO sums.
+//
+// The moments are wave-uniform and come in through scalar loads.  They are
+// consumed in groups of a few channels x CH terms, fenced by scheduling
+// barriers, so only ~32 of them are live in SGPRs at a time (without the
+// fences the scheduler hoists every load of the node and the SGPRs spill).
+template <int KID>
+struct TreeChunk {
+    static constexpr int CH = KID == TCT_RT ? 4 : 8;
+};
+
 template <int KID, int P, typename T>
 __device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__restrict__ M,
                                          T inv_rho, const T mn[3], const KConst<T> &k, T *out) {
     constexpr int TT = n_terms(P);
-    constexpr int C = TreeK<KID>::C;
-    (void)C;
+    constexpr int CH = TreeChunk<KID>::CH;
+    constexpr int NCH = (TT + CH - 1) / CH;
     // t_k(I) = (a_k + 1) a_{I + e_k}: the x_k-gradient weight of term I
     auto tk = [&](auto ic, int kk) -> T {
         constexpr int I = decltype(ic)::value;
@@ -272,24 +305,30 @@ __device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__r
     const T ir2 = inv_rho * inv_rho;
     if constexpr (KID == TCT_U || KID == TCT_RA) {
         T phi[3] = {0, 0, 0}, g[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-        static_for<0, TT>([&](auto ic) {
-            constexpr int I = decltype(ic)::value;
-            const T t0 = tk(ic, 0), t1 = tk(ic, 1), t2 = tk(ic, 2);
-            const T m0 = M[0 * TT + I], m1 = M[1 * TT + I], m2 = M[2 * TT + I];
-            phi[0] += m0 * a[I];
-            phi[1] += m1 * a[I];
-            phi[2] += m2 * a[I];
-            const T nI = M[3 * TT + I] + Rt[0] * m0 + Rt[1] * m1 + Rt[2] * m2;
-            g[0] += nI * t0;
-            g[1] += nI * t1;
-            g[2] += nI * t2;
-            if constexpr (KID == TCT_RA) {
-                const T mt = mn[0] * t0 + mn[1] * t1 + mn[2] * t2;
-                h[0] += m0 * mt;
-                h[1] += m1 * mt;
-                h[2] += m2 * mt;
-            }
+        static_for<0, NCH>([&](auto cc) {
+            constexpr int I0 = decltype(cc)::value * CH;
+            constexpr int I1 = I0 + CH < TT ? I0 + CH : TT;
+            __builtin_amdgcn_sched_barrier(0);
+            static_for<I0, I1>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                const T t0 = tk(ic, 0), t1 = tk(ic, 1), t2 = tk(ic, 2);
+                const T m0 = M[0 * TT + I], m1 = M[1 * TT + I], m2 = M[2 * TT + I];
+                phi[0] += m0 * a[I];
+                phi[1] += m1 * a[I];
+                phi[2] += m2 * a[I];
+                const T nI = M[3 * TT + I] + Rt[0] * m0 + Rt[1] * m1 + Rt[2] * m2;
+                g[0] += nI * t0;
+                g[1] += nI * t1;
+                g[2] += nI * t2;
+                if constexpr (KID == TCT_RA) {
+                    const T mt = mn[0] * t0 + mn[1] * t1 + mn[2] * t2;
+                    h[0] += m0 * mt;
+                    h[1] += m1 * mt;
+                    h[2] += m2 * mt;
+                }
+            });
         });
+        __builtin_amdgcn_sched_barrier(0);
         if constexpr (KID == TCT_U) {
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk) out[kk] += inv_rho * k.c0 * phi[kk] - ir2 * g[kk];
@@ -303,21 +342,35 @@ __device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__r
         }
     } else if constexpr (KID == TCT_RT) {
         T phi[3] = {0, 0, 0}, g[3] = {0, 0, 0}, h[3] = {0, 0, 0};
-        static_for<0, TT>([&](auto ic) {
-            constexpr int I = decltype(ic)::value;
-            const T t[3] = {tk(ic, 0), tk(ic, 1), tk(ic, 2)};
-            const T m0 = M[0 * TT + I], m1 = M[1 * TT + I], m2 = M[2 * TT + I];
-            phi[0] += m0 * a[I];
-            phi[1] += m1 * a[I];
-            phi[2] += m2 * a[I];
-            const T nI = M[3 * TT + I] + Rt[0] * m0 + Rt[1] * m1 + Rt[2] * m2;
-#pragma unroll
-            for (int kk = 0; kk < 3; ++kk) g[kk] += nI * t[kk];
-#pragma unroll
-            for (int aa = 0; aa < 3; ++aa)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) h[aa] += M[(4 + 3 * aa + j) * TT + I] * t[j];
+        static_for<0, NCH>([&](auto cc) {
+            constexpr int I0 = decltype(cc)::value * CH;
+            constexpr int I1 = I0 + CH < TT ? I0 + CH : TT;
+            // e, Psi_e channels
+            __builtin_amdgcn_sched_barrier(0);
+            static_for<I0, I1>([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                const T m0 = M[0 * TT + I], m1 = M[1 * TT + I], m2 = M[2 * TT + I];
+                phi[0] += m0 * a[I];
+                phi[1] += m1 * a[I];
+                phi[2] += m2 * a[I];
+                const T nI = M[3 * TT + I] + Rt[0] * m0 + Rt[1] * m1 + Rt[2] * m2;
+                g[0] += nI * tk(ic, 0);
+                g[1] += nI * tk(ic, 1);
+                g[2] += nI * tk(ic, 2);
+            });
+            // s_a n_j channels, one a at a time
+            static_for<0, 3>([&](auto ac) {
+                constexpr int aa = decltype(ac)::value;
+                __builtin_amdgcn_sched_barrier(0);
+                static_for<I0, I1>([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    h[aa] += M[(4 + 3 * aa + 0) * TT + I] * tk(ic, 0) +
+                             M[(4 + 3 * aa + 1) * TT + I] * tk(ic, 1) +
+                             M[(4 + 3 * aa + 2) * TT + I] * tk(ic, 2);
+                });
+            });
         });
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk)
             out[kk] += inv_rho * phi[kk] - ir2 * (k.c0 * g[kk] + h[kk]);
@@ -327,21 +380,27 @@ __device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__r
         T phi[9], b[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) phi[q] = b[q] = T(0);
-        static_for<0, TT>([&](auto ic) {
-            constexpr int I = decltype(ic)::value;
-            const T t[3] = {tk(ic, 0), tk(ic, 1), tk(ic, 2)};
-#pragma unroll
-            for (int mm = 0; mm < 3; ++mm) {
-                const T g0 = M[(0 + mm) * TT + I], g1 = M[(3 + mm) * TT + I],
-                        g2 = M[(6 + mm) * TT + I];
-                phi[0 + mm] += g0 * a[I];
-                phi[3 + mm] += g1 * a[I];
-                phi[6 + mm] += g2 * a[I];
-                const T nI = M[(9 + mm) * TT + I] + Rt[0] * g0 + Rt[1] * g1 + Rt[2] * g2;
-#pragma unroll
-                for (int kk = 0; kk < 3; ++kk) b[kk * 3 + mm] += nI * t[kk];
-            }
+        static_for<0, NCH>([&](auto cc) {
+            constexpr int I0 = decltype(cc)::value * CH;
+            constexpr int I1 = I0 + CH < TT ? I0 + CH : TT;
+            static_for<0, 3>([&](auto mc) {
+                constexpr int mm = decltype(mc)::value;
+                __builtin_amdgcn_sched_barrier(0);
+                static_for<I0, I1>([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    const T g0 = M[(0 + mm) * TT + I], g1 = M[(3 + mm) * TT + I],
+                            g2 = M[(6 + mm) * TT + I];
+                    phi[0 + mm] += g0 * a[I];
+                    phi[3 + mm] += g1 * a[I];
+                    phi[6 + mm] += g2 * a[I];
+                    const T nI = M[(9 + mm) * TT + I] + Rt[0] * g0 + Rt[1] * g1 + Rt[2] * g2;
+                    b[0 * 3 + mm] += nI * tk(ic, 0);
+                    b[1 * 3 + mm] += nI * tk(ic, 1);
+                    b[2 * 3 + mm] += nI * tk(ic, 2);
+                });
+            });
         });
+        __builtin_amdgcn_sched_barrier(0);
         const T tr = phi[0] + phi[4] + phi[8];
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk)
@@ -354,25 +413,33 @@ __device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__r
     }
 }
 
-// Evaluation: one wave (block of 64) per observation tile, grid.y chunks.
+// Evaluation: one wave (block of 64) per list segment.  seg [n_seg][3] int32
+// = (observation tile, first entry, end entry) of `list`; each wave sums its
+// segment for the tile's 64 points into seg_partial [n_seg][NO][64] (f64),
+// and tree_reduce_kernel adds a tile's segments in order.
 // FP32 (ABS = false): obs = loc32 [6][n_obs_pad] local to obs_org per tile,
 // records local to src_org per tile.  FP64 (ABS = true): obs = soa64
-// absolute, records absolute.
+// absolute, records absolute.  p2p source records are read with scalar
+// loads (wave-uniform, as in far_p2p.hip); an m2p node's moments (C x TT
+// values, too many for SGPRs) are copied by the wave into LDS and read back
+// as broadcasts.
 template <int KID, int P, typename T, bool ABS>
 __global__ __launch_bounds__(64) void tree_eval_kernel(
     int n_obs_pad, const T *__restrict__ obs, const double *__restrict__ obs_org,
     const T *__restrict__ rec, const double *__restrict__ src_org,
     const double *__restrict__ node_cr, const T *__restrict__ moments,
-    const int *__restrict__ list_start, const int *__restrict__ list, KConst<T> k,
-    double *__restrict__ partial) {
+    const int *__restrict__ seg, const int *__restrict__ list, KConst<T> k,
+    double *__restrict__ seg_partial) {
     constexpr int RW = rec_words<KID>();
     constexpr int NO = n_out<KID>();
     constexpr int C = TreeK<KID>::C;
     constexpr int TT = n_terms(P);
-    const int tile = blockIdx.x;
-    const int chunks = gridDim.y;
-    const int cidx = blockIdx.y;
-    const int i = tile * G + (int)threadIdx.x;
+    constexpr int NM = C * TT;
+    __shared__ T smom[NM];
+    const int sg = blockIdx.x;
+    const int tile = seg[3 * sg];
+    const int lane = (int)threadIdx.x;
+    const int i = tile * G + lane;
 
     T x[3], mn[3];
     double oo[3];
@@ -387,10 +454,7 @@ __global__ __launch_bounds__(64) void tree_eval_kernel(
 #pragma unroll
     for (int q = 0; q < NO; ++q) acc64[q] = 0.0;
 
-    const int e0 = list_start[tile], e1 = list_start[tile + 1];
-    const int len = e1 - e0;
-    const int b0 = e0 + (int)(((long)len * cidx) / chunks);
-    const int b1 = e0 + (int)(((long)len * (cidx + 1)) / chunks);
+    const int b0 = seg[3 * sg + 1], b1 = seg[3 * sg + 2];
     for (int e = b0; e < b1; ++e) {
         const int code = list[e];
         T acc[NO];
@@ -406,6 +470,14 @@ __global__ __launch_bounds__(64) void tree_eval_kernel(
 #pragma unroll 2
             for (int j = 0; j < G; ++j) pair_eval<KID, T>(xs, mn, r + j * RW, k, acc);
         } else {
+            const T *Mg = moments + (size_t)code * NM;
+            constexpr int NL = (NM + G - 1) / G;
+            T mv[NL];
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                const int idx = l * G + lane;
+                mv[l] = idx < NM ? Mg[idx] : T(0);
+            }
             const double *cr = node_cr + 4 * (size_t)code;
             T Rt[3], u[3];
             const T inv_rho = (T)(1.0 / cr[3]);
@@ -416,31 +488,38 @@ __global__ __launch_bounds__(64) void tree_eval_kernel(
             }
             T a[n_terms(P + 1)];
             taylor_derivs<P + 1, T>(u, a);
-            m2p_eval<KID, P, T>(a, Rt, moments + (size_t)code * C * TT, inv_rho, mn, k, acc);
+#pragma unroll
+            for (int l = 0; l < NL; ++l) {
+                const int idx = l * G + lane;
+                if (idx < NM) smom[idx] = mv[l];
+            }
+            __syncthreads();
+            m2p_eval<KID, P, T>(a, Rt, smom, inv_rho, mn, k, acc);
+            __syncthreads();
         }
 #pragma unroll
         for (int q = 0; q < NO; ++q) acc64[q] += (double)acc[q];
     }
-    double *out = partial + (size_t)cidx * NO * n_obs_pad;
+    double *out = seg_partial + (size_t)sg * NO * G;
 #pragma unroll
-    for (int q = 0; q < NO; ++q) out[(size_t)q * n_obs_pad + i] = acc64[q];
+    for (int q = 0; q < NO; ++q) out[q * G + lane] = acc64[q];
 }
 
 
 template <int KID, int P>
 inline void launch_eval(int prec, dim3 grid, hipStream_t st, int n_obs_pad, const void *obs,
                         const double *obs_org, const void *rec, const double *src_org,
-                        const double *node_cr, const void *moments, const int *list_start,
+                        const double *node_cr, const void *moments, const int *seg,
                         const int *list, const double *params, double eps2, double *partial) {
     if (prec == TCT_F32)
         hipLaunchKernelGGL((tree_eval_kernel<KID, P, float, false>), grid, dim3(G), 0, st,
                            n_obs_pad, (const float *)obs, obs_org, (const float *)rec, src_org,
-                           node_cr, (const float *)moments, list_start, list,
+                           node_cr, (const float *)moments, seg, list,
                            kernel_consts<float>(KID, params, eps2), partial);
     else
         hipLaunchKernelGGL((tree_eval_kernel<KID, P, double, true>), grid, dim3(G), 0, st,
                            n_obs_pad, (const double *)obs, obs_org, (const double *)rec, src_org,
-                           node_cr, (const double *)moments, list_start, list,
+                           node_cr, (const double *)moments, seg, list,
                            kernel_consts<double>(KID, params, 0.0), partial);
 }
 
@@ -448,11 +527,11 @@ template <int KID>
 inline int eval_order(int order, int prec, dim3 grid, hipStream_t st, int n_obs_pad,
                       const void *obs, const double *obs_org, const void *rec,
                       const double *src_org, const double *node_cr, const void *moments,
-                      const int *list_start, const int *list, const double *params, double eps2,
+                      const int *seg, const int *list, const double *params, double eps2,
                       double *partial) {
 #define TCT_EV(P)                                                                              \
     launch_eval<KID, P>(prec, grid, st, n_obs_pad, obs, obs_org, rec, src_org, node_cr,        \
-                        moments, list_start, list, params, eps2, partial)
+                        moments, seg, list, params, eps2, partial)
     switch (order) {
     case 2: TCT_EV(2); break;
     case 3: TCT_EV(3); break;
@@ -472,35 +551,35 @@ template <int KID>
 int tree_eval_kernel_entry(int order, int prec, dim3 grid, hipStream_t st, int n_obs_pad,
                            const void *obs, const double *obs_org, const void *rec,
                            const double *src_org, const double *node_cr,
-                           const void *moments, const int *list_start, const int *list,
+                           const void *moments, const int *seg, const int *list,
                            const double *params, double eps2, double *partial);
 template <>
 int tree_eval_kernel_entry<TCT_U>(int order, int prec, dim3 grid, hipStream_t st,
                                 int n_obs_pad, const void *obs, const double *obs_org,
                                 const void *rec, const double *src_org,
                                 const double *node_cr, const void *moments,
-                                const int *list_start, const int *list,
+                                const int *seg, const int *list,
                                 const double *params, double eps2, double *partial);
 template <>
 int tree_eval_kernel_entry<TCT_RT>(int order, int prec, dim3 grid, hipStream_t st,
                                 int n_obs_pad, const void *obs, const double *obs_org,
                                 const void *rec, const double *src_org,
                                 const double *node_cr, const void *moments,
-                                const int *list_start, const int *list,
+                                const int *seg, const int *list,
                                 const double *params, double eps2, double *partial);
 template <>
 int tree_eval_kernel_entry<TCT_RA>(int order, int prec, dim3 grid, hipStream_t st,
                                 int n_obs_pad, const void *obs, const double *obs_org,
                                 const void *rec, const double *src_org,
                                 const double *node_cr, const void *moments,
-                                const int *list_start, const int *list,
+                                const int *seg, const int *list,
                                 const double *params, double eps2, double *partial);
 template <>
 int tree_eval_kernel_entry<TCT_RH>(int order, int prec, dim3 grid, hipStream_t st,
                                 int n_obs_pad, const void *obs, const double *obs_org,
                                 const void *rec, const double *src_org,
                                 const double *node_cr, const void *moments,
-                                const int *list_start, const int *list,
+                                const int *seg, const int *list,
                                 const double *params, double eps2, double *partial);
 
 }  // namespace tct

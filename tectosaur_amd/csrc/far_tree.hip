@@ -2,6 +2,26 @@
 // formulation and layout are described in far_tree.h.
 #include "far_tree.h"
 
+namespace tct {
+
+// out [NO][n_obs_pad] = sum over each tile's segments [seg_off[t],
+// seg_off[t + 1]) of seg_partial, in segment order.  grid = n_obs_tiles,
+// one thread per (output q, point).
+__global__ __launch_bounds__(576) void tree_reduce_kernel(int no, int n_obs_pad,
+                                                          const int *__restrict__ seg_off,
+                                                          const double *__restrict__ seg_partial,
+                                                          double *__restrict__ out) {
+    const int t = blockIdx.x;
+    const int q = threadIdx.x / G, lane = threadIdx.x % G;
+    if (q >= no) return;
+    const int s0 = seg_off[t], s1 = seg_off[t + 1];
+    double v = 0.0;
+    for (int s = s0; s < s1; ++s) v += seg_partial[((size_t)s * no + q) * G + lane];
+    out[(size_t)q * n_obs_pad + t * G + lane] = v;
+}
+
+}  // namespace tct
+
 using namespace tct;
 
 static int tree_kernel_ok(int kernel) {
@@ -70,27 +90,25 @@ extern "C" int tct_tree_m2m(int kernel, int prec, int order, int p_off, int n_pa
     return TCT_OK;
 }
 
-extern "C" int tct_tree_eval(int kernel, int prec, int order, int n_obs_tiles, int n_obs_pad,
-                             const void *obs, const double *obs_org, const void *rec,
-                             const double *src_org, const double *node_cr, const void *moments,
-                             const int *list_start, const int *list, const double *params_host,
-                             double eps2, int chunks, double *out_partial, void *stream) {
+extern "C" int tct_tree_eval(int kernel, int prec, int order, int n_obs_pad, const void *obs,
+                             const double *obs_org, const void *rec, const double *src_org,
+                             const double *node_cr, const void *moments, int n_seg,
+                             const int *seg, const int *list, const double *params_host,
+                             double eps2, double *seg_partial, void *stream) {
     TCT_CHECK_ARG(tree_kernel_ok(kernel), "tree_eval: kernel %d has no treecode form", kernel);
     TCT_CHECK_ARG(prec == TCT_F32 || prec == TCT_F64, "tree_eval: precision %d", prec);
-    TCT_CHECK_ARG(n_obs_tiles > 0 && n_obs_pad == n_obs_tiles * G,
-                  "tree_eval: n_obs_pad %d must be 64 * n_obs_tiles (%d)", n_obs_pad,
-                  n_obs_tiles);
-    TCT_CHECK_ARG(chunks >= 1 && chunks <= 65535, "tree_eval: chunks %d", chunks);
-    TCT_CHECK_ARG(obs && rec && node_cr && moments && list_start && list && params_host &&
-                      out_partial && (prec == TCT_F64 || (obs_org && src_org)),
+    TCT_CHECK_ARG(n_obs_pad > 0 && n_obs_pad % G == 0, "tree_eval: n_obs_pad %d", n_obs_pad);
+    TCT_CHECK_ARG(n_seg > 0, "tree_eval: n_seg %d", n_seg);
+    TCT_CHECK_ARG(obs && rec && node_cr && moments && seg && list && params_host &&
+                      seg_partial && (prec == TCT_F64 || (obs_org && src_org)),
                   "tree_eval: null pointer");
     hipStream_t st = as_stream(stream);
-    const dim3 grid((unsigned)n_obs_tiles, (unsigned)chunks);
+    const dim3 grid((unsigned)n_seg);
     int rc;
 #define TCT_EK(KID)                                                                           \
     rc = tree_eval_kernel_entry<KID>(order, prec, grid, st, n_obs_pad, obs, obs_org, rec,     \
-                                     src_org, node_cr, moments, list_start, list, params_host, \
-                                     eps2, out_partial)
+                                     src_org, node_cr, moments, seg, list, params_host, eps2, \
+                                     seg_partial)
     switch (kernel) {
     case TCT_U: TCT_EK(TCT_U); break;
     case TCT_RT: TCT_EK(TCT_RT); break;
@@ -100,5 +118,19 @@ extern "C" int tct_tree_eval(int kernel, int prec, int order, int n_obs_tiles, i
 #undef TCT_EK
     if (rc != TCT_OK) return rc;
     TCT_CHECK_LAUNCH("tree_eval");
+    return TCT_OK;
+}
+
+extern "C" int tct_tree_reduce(int kernel, int n_obs_tiles, int n_obs_pad, const int *seg_off,
+                               const double *seg_partial, double *out_partial, void *stream) {
+    TCT_CHECK_ARG(tree_kernel_ok(kernel), "tree_reduce: kernel %d has no treecode form", kernel);
+    TCT_CHECK_ARG(n_obs_tiles > 0 && n_obs_pad == n_obs_tiles * G,
+                  "tree_reduce: n_obs_pad %d must be 64 * n_obs_tiles (%d)", n_obs_pad,
+                  n_obs_tiles);
+    TCT_CHECK_ARG(seg_off && seg_partial && out_partial, "tree_reduce: null pointer");
+    const int no = (kernel == TCT_RA || kernel == TCT_RH) ? 9 : 3;
+    hipLaunchKernelGGL(tree_reduce_kernel, dim3((unsigned)n_obs_tiles), dim3(G * no), 0,
+                       as_stream(stream), no, n_obs_pad, seg_off, seg_partial, out_partial);
+    TCT_CHECK_LAUNCH("tree_reduce");
     return TCT_OK;
 }

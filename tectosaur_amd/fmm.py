@@ -153,8 +153,23 @@ def obs_tile_cr(pts, n_pad):
     return np.concatenate([c, r[:, None]], axis=1)
 
 
-def default_chunks(n_obs_tiles, n_cu=256, waves_per_cu=8):
-    return int(max(1, min(64, -(-waves_per_cu * n_cu // n_obs_tiles))))
+def segments(start, n_waves):
+    """Split every tile's list [start[t], start[t+1]) into segments of at
+    most S entries, S chosen so that about ``n_waves`` segments result
+    (bounded to [2, 64]).  Returns (seg [n_seg, 3] int32 = (tile, lo, hi),
+    seg_off [n_tiles + 1] int32)."""
+    start = np.asarray(start, dtype=np.int64)
+    lens = np.diff(start)
+    S = int(min(64, max(2, -(-int(start[-1]) // max(n_waves, 1)))))
+    nseg = np.maximum(1, -(-lens // S))
+    seg_off = np.zeros(lens.shape[0] + 1, dtype=np.int64)
+    seg_off[1:] = np.cumsum(nseg)
+    tile = np.repeat(np.arange(lens.shape[0]), nseg)
+    k = np.arange(int(seg_off[-1])) - seg_off[tile]
+    lo = start[tile] + k * S
+    hi = np.minimum(lo + S, start[tile + 1])
+    seg = np.stack([tile, lo, hi], axis=1).astype(np.int32)
+    return np.ascontiguousarray(seg), seg_off.astype(np.int32), S
 
 
 def tree_plan(obs_pts, src_pts, mac=3.0, pts_per_cell=64):
@@ -174,7 +189,7 @@ class TreeFarfield:
     same partial-sum output)."""
 
     def __init__(self, engine, obs_pts=None, src_pts=None, order=4, mac=3.0, pts_per_cell=64,
-                 chunks=None, plan=None):
+                 plan=None, n_waves=None):
         if engine.K.name not in TREE_KERNELS:
             raise NotImplementedError('%s has no treecode form (U3, RT3, RA3, RH3 do)'
                                       % engine.K.name)
@@ -194,17 +209,22 @@ class TreeFarfield:
             raise ValueError('treecode plan does not match the engine layout')
         dev = engine.device
         self.n_obs_tiles = engine.n_obs_pad // GROUP
-        self.list_start = torch.as_tensor(start, device=dev)
         self.list = torch.as_tensor(lst, device=dev)
         self.node_cr = torch.as_tensor(self.tree.node_cr, dtype=torch.float64, device=dev)
         self.C = N_CHANNELS[engine.K.name]
         self.T = n_terms(order)
-        if chunks is None:
+        if n_waves is None:
+            # enough wavefronts to hide the latency of each wave's serial list
             n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
-            chunks = default_chunks(self.n_obs_tiles, n_cu)
-        self.chunks = int(chunks)
+            n_waves = 64 * n_cu
+        seg, seg_off, self.seg_len = segments(start, n_waves)
+        self.n_seg = seg.shape[0]
+        self.seg = torch.as_tensor(seg, device=dev)
+        self.seg_off = torch.as_tensor(seg_off, device=dev)
+        self.chunks = 1  # the reduced output has one chunk
         self._moments = {}
         self._partial = None
+        self._seg_partial = None
 
     def moments(self, prec):
         if prec not in self._moments:
@@ -216,8 +236,10 @@ class TreeFarfield:
     @property
     def partial(self):
         if self._partial is None:
-            self._partial = torch.empty(self.chunks * self.engine.n_out * self.engine.n_obs_pad,
+            self._partial = torch.empty(self.engine.n_out * self.engine.n_obs_pad,
                                         dtype=torch.float64, device=self.engine.device)
+            self._seg_partial = torch.empty(self.n_seg * self.engine.n_out * GROUP,
+                                            dtype=torch.float64, device=self.engine.device)
         return self._partial
 
     def run(self, prec='f32'):
@@ -236,11 +258,14 @@ class TreeFarfield:
             N.call('tct_tree_m2m', e.kid, pid, self.order, p_off, n_par, c_off, n_child,
                    N.ptr(self.node_cr), N.ptr(mom), st)
         obs = e.loc32 if prec == 'f32' else e.obs_soa64
-        N.call('tct_tree_eval', e.kid, pid, self.order, self.n_obs_tiles, e.n_obs_pad,
-               N.ptr(obs), N.ptr(e.obs_org), N.ptr(rec), N.ptr(e.src_org), N.ptr(self.node_cr),
-               N.ptr(mom), N.ptr(self.list_start), N.ptr(self.list), e.params.ctypes.data,
-               float(e.eps2), self.chunks, N.ptr(self.partial), st)
-        return self.partial
+        out = self.partial
+        N.call('tct_tree_eval', e.kid, pid, self.order, e.n_obs_pad, N.ptr(obs), N.ptr(e.obs_org),
+               N.ptr(rec), N.ptr(e.src_org), N.ptr(self.node_cr), N.ptr(mom), self.n_seg,
+               N.ptr(self.seg), N.ptr(self.list), e.params.ctypes.data, float(e.eps2),
+               N.ptr(self._seg_partial), st)
+        N.call('tct_tree_reduce', e.kid, self.n_obs_tiles, e.n_obs_pad, N.ptr(self.seg_off),
+               N.ptr(self._seg_partial), N.ptr(out), st)
+        return out
 
     def pair_evals_equivalent(self):
         """Direct pair evaluations done by the p2p pass (64 x 64 per tile pair)."""

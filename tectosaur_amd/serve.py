@@ -96,8 +96,6 @@ def _prepare(req, orders, tree=None):
 
 def _build(prep, mode, tol, restart, maxiter):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
-    if mode == 'tree' and prep.get('tree') is None:
-        raise ValueError("mode 'tree' needs requests prepared with a treecode")
     prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
     b = op.cmT.dot_dev(prob.rhs)

@@ -117,14 +117,15 @@ class TriToTriDirectFarfieldOp:
         self._plan_pts = (plan['o_pts'], plan['s_pts'])
         self.tree = None
 
-    def attach_tree(self, order=4, mac=3.0, pts_per_cell=64, chunks=None, plan=None):
+    def attach_tree(self, order=4, mac=3.0, pts_per_cell=64, plan=None, n_waves=None):
         """Add a treecode evaluation of this operator (kernels U3, RT3, RA3,
         RH3), used by ``dot_dev(..., prec='tree')`` (FP32) and
         ``tree_dot_dev``; the direct FP32/FP64 paths stay available.
-        ``plan``: a precomputed ``fmm.tree_plan`` for these point sets."""
+        ``plan``: a precomputed ``fmm.tree_plan`` for these point sets;
+        ``n_waves``: target number of list segments (wavefronts)."""
         from .fmm import TreeFarfield
         self.tree = TreeFarfield(self.engine, self._plan_pts[0], self._plan_pts[1], order=order,
-                                 mac=mac, pts_per_cell=pts_per_cell, chunks=chunks, plan=plan)
+                                 mac=mac, pts_per_cell=pts_per_cell, plan=plan, n_waves=n_waves)
         return self.tree
 
     def tree_dot_dev(self, v_dev, out_dev=None, prec='f32'):
@@ -162,11 +163,13 @@ class TriToTriDirectFarfieldOp:
 
     def dot_dev(self, v_dev, out_dev=None, prec=None):
         """Device matvec: v_dev float64 [n_src*9] -> float64 [n_obs*9].
-        prec: 'f32' or 'f64' (direct N-body), or 'tree' (FP32 treecode, after
-        attach_tree)."""
+        prec: 'f32' or 'f64' (direct N-body), or 'tree' (FP32 treecode when
+        one is attached, else the FP32 direct N-body)."""
         prec = prec or self.prec
         if prec == 'tree':
-            return self.tree_dot_dev(v_dev, out_dev, 'f32')
+            if self.tree is not None:
+                return self.tree_dot_dev(v_dev, out_dev, 'f32')
+            prec = 'f32'
         if out_dev is None:
             out_dev = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
         self._gather(v_dev, prec)

@@ -118,7 +118,10 @@ def fault_mesh(req):
     return mesh_mod.make_rect(nx + 1, nz + 1, corners)
 
 
-DEFAULT_TREE = dict(order=4, mac=2.5)
+# treecode of the served 'tree' mode: expansion order, acceptance parameter
+# and the smallest problem (triangles) that gets one -- below it the direct
+# FP32 N-body is faster than the treecode's fixed per-matvec passes
+DEFAULT_TREE = dict(order=4, mac=2.5, min_tris=8192)
 
 
 def prepare_problem(pts, tris, traction, sm=3e10, orders=None, K_far='elasticRH3', tree=None):
@@ -142,7 +145,7 @@ def prepare_problem(pts, tris, traction, sm=3e10, orders=None, K_far='elasticRH3
     out = dict(pts=pts, tris=tris, orders=o, K_far=K_far,
                near_plan=nearfield_plan(pts, tris, idx, idx, o['near_threshold']),
                far_plan=far_plan, M=M, cm=continuity_cm(tris), rhs=-M.dot(t / sm), tree=None)
-    if tree is not None:
+    if tree is not None and tris.shape[0] >= tree.get('min_tris', 0):
         from .fmm import tree_plan
         out['tree'] = dict(tree, plan=tree_plan(far_plan['o_pts'], far_plan['s_pts'],
                                                 tree['mac']))

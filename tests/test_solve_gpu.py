@@ -106,11 +106,12 @@ def test_tree_server_result_same_alone_and_in_mix():
     from tectosaur_amd.serve import Server
     from tectosaur_amd.traction_slip import random_requests
     reqs = random_requests(4, rate=0.0, seed=8, n_min=400, n_max=1500)
-    mix = Server(mode='tree', max_active=4, restart=100, orders=ORD, keep_slip=True)
+    tree = dict(order=4, mac=2.5, min_tris=0)
+    mix = Server(mode='tree', max_active=4, restart=100, orders=ORD, keep_slip=True, tree=tree)
     recs_mix = mix.run(reqs, realtime=False)
     for req, rm in zip(reqs, recs_mix):
-        ra = Server(mode='tree', max_active=1, restart=100, orders=ORD,
-                    keep_slip=True).run([req], realtime=False)[0]
+        ra = Server(mode='tree', max_active=1, restart=100, orders=ORD, keep_slip=True,
+                    tree=tree).run([req], realtime=False)[0]
         assert rm.status == ra.status == 'converged' and rm.rel_residual <= 1e-8
         assert rm.iterations == ra.iterations
         assert np.abs(rm.slip - ra.slip).max() <= 1e-12 * np.abs(ra.slip).max()
