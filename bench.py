@@ -200,7 +200,7 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512):
     warm = random_requests(2, rate=0.0, seed=seed + 99, n_min=512, n_max=1024)
     Server(mode='tree', max_active=2).run(warm, realtime=False)
     Server(mode='mixed', max_active=2).run(warm, realtime=False)
-    Server(mode='f64', max_active=1, prep_workers=0).run(warm, realtime=False)
+    Server(mode='f64', max_active=1, prep_workers=0, graphs=False).run(warm, realtime=False)
     reqs = random_requests(n_req, rate=rate, seed=seed, n_min=min_tris, n_max=max_tris)
     out = {'workload': '%d fault requests, Poisson %.2f/s, strike U(0,360) dip U(20,90) '
                        'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near and'
@@ -208,7 +208,7 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512):
     for name, mode, ma, pw in (('batched_tree', 'tree', max_active, 4),
                                ('batched_mixed', 'mixed', max_active, 4),
                                ('sequential_f64', 'f64', 1, 0)):
-        srv = Server(mode=mode, max_active=ma, prep_workers=pw)
+        srv = Server(mode=mode, max_active=ma, prep_workers=pw, graphs=(ma > 1))
         torch.cuda.synchronize()
         recs = srv.run(reqs, realtime=True)
         sm = summarize(recs, srv.makespan)

@@ -54,6 +54,7 @@ SIGNATURES = {
     'tct_far_reduce_pts': [_i64, _i64, _int, _vp, _vp, _vp],
     'tct_tree_p2m': [_int, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     'tct_tree_m2m': [_int, _int, _int, _int, _int, _int, _int, _vp, _vp, _vp],
+    'tct_tree_m2m_multi': [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp],
     'tct_tree_eval': [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _vp,
                       _vp, _dbl, _vp, _vp],
     'tct_tree_reduce': [_int, _int, _int, _vp, _vp, _vp, _vp],

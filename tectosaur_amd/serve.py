@@ -29,7 +29,7 @@ import time
 import numpy as np
 import torch
 
-from .solver import ConstrainedOperator, make_state
+from .solver import MODES, ConstrainedOperator, make_state
 from .traction_slip import (DEFAULT_TREE, TractionSlipProblem, fault_mesh, prepare_problem,
                             request_traction)
 
@@ -94,12 +94,14 @@ def _prepare(req, orders, tree=None):
     return prep
 
 
-def _build(prep, mode, tol, restart, maxiter):
+def _build(prep, mode, tol, restart, maxiter, graphs=False):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
     prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
     b = op.cmT.dot_dev(prob.rhs)
     state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
+    if graphs:
+        op.enable_graphs(MODES[mode][0:2] + MODES[mode][3:4])
     return prob, op, state
 
 
@@ -117,7 +119,8 @@ def _finalize(a, now, keep_slip):
 
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
-                 hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None):
+                 hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None,
+                 graphs=True):
         self.mode = mode
         self.max_active = max_active
         self.tol = tol
@@ -129,6 +132,8 @@ class Server:
         self.tree = (tree or dict(DEFAULT_TREE)) if mode == 'tree' else None
         self.keep_slip = keep_slip
         self.prep_workers = prep_workers
+        # capture each solve's operator applications into HIP graphs
+        self.graphs = graphs
         self.ticks = 0
 
     def run(self, requests, realtime=True, log=None):
@@ -184,7 +189,7 @@ class Server:
                         prep = _prepare(req, self.orders, self.tree)
                     rec.prep_s = prep['prep_s']
                     prob, op, state = _build(prep, self.mode, self.tol, self.restart,
-                                             self.maxiter)
+                                             self.maxiter, self.graphs)
                     rec.n_tris = prob.n_tris
                     rec.pair_evals = prob.pair_evals
                     rec.setup_s = clock() - rec.admitted
@@ -226,7 +231,8 @@ class Server:
 
 def serve_sequential(requests, mode='f64', realtime=True, **kw):
     """The one-at-a-time path: each request prepared, built and solved alone,
-    in arrival order, with no overlap."""
+    in arrival order, with no overlap and no graph capture."""
+    kw.setdefault('graphs', False)
     return Server(mode=mode, max_active=1, prep_workers=0, **kw).run(requests, realtime=realtime)
 
 

@@ -43,13 +43,43 @@ class ConstrainedOperator:
         self._u = torch.empty(cm.shape[0], dtype=torch.float64, device=self.device)
         self._y = torch.empty(cm.shape[0], dtype=torch.float64, device=self.device)
         self.n_matvec = {'f32': 0, 'f64': 0, 'tree': 0}
+        self._graphs = {}
+
+    def _apply_raw(self, x, out, prec):
+        self.cm.dot_dev(x, self._u)
+        self.A.dot_dev(self._u, self._y, prec)
+        self.cmT.dot_dev(self._y, out)
+
+    def enable_graphs(self, precs):
+        """Capture the whole apply (CSR, farfield passes, BSR, CSR) for each
+        precision in ``precs`` into a HIP graph replayed by ``apply``: one
+        graph launch and two vector copies per application instead of a
+        dozen kernel launches.  The captured kernels write every output in
+        full (no memsets) and use only buffers the operator owns."""
+        n = self.shape[0]
+        self._gx = torch.zeros(n, dtype=torch.float64, device=self.device)
+        self._gy = torch.empty(n, dtype=torch.float64, device=self.device)
+        for prec in precs:
+            if prec is None or prec in self._graphs:
+                continue
+            self._apply_raw(self._gx, self._gy, prec)      # allocate lazy buffers eagerly
+            torch.cuda.synchronize(self.device)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._apply_raw(self._gx, self._gy, prec)
+            self._graphs[prec] = g
+        torch.cuda.synchronize(self.device)
 
     def apply(self, x, out=None, prec='f64'):
         if out is None:
             out = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
-        self.cm.dot_dev(x, self._u)
-        self.A.dot_dev(self._u, self._y, prec)
-        self.cmT.dot_dev(self._y, out)
+        g = self._graphs.get(prec)
+        if g is not None:
+            self._gx.copy_(x)
+            g.replay()
+            out.copy_(self._gy)
+        else:
+            self._apply_raw(x, out, prec)
         self.n_matvec[prec] += 1
         return out
 
@@ -87,7 +117,8 @@ class GmresState:
     does), or interleave many states with one transfer per tick."""
 
     def __init__(self, apply, b, tol=1e-8, restart=200, maxiter=2000, inner_prec='f64',
-                 outer_prec='f64', inner_reduction=None, x0=None, callback=None):
+                 outer_prec='f64', inner_reduction=None, x0=None, callback=None,
+                 coarse_prec=None, coarse_above=1e-4):
         self.apply = apply
         self.b = b
         dev = b.device
@@ -99,6 +130,13 @@ class GmresState:
         self.inner_prec = inner_prec
         self.outer_prec = outer_prec
         self.inner_reduction = inner_reduction
+        # residuals expected above coarse_above * ||b|| may be computed in
+        # coarse_prec (a cheaper operator accurate to well below that level);
+        # convergence is only ever declared from an outer_prec residual
+        self.coarse_prec = coarse_prec
+        self.coarse_above = coarse_above
+        self._expect = float('inf')
+        self._res_prec = outer_prec
         self.callback = callback
         self.x = torch.zeros(n, dtype=torch.float64, device=dev) if x0 is None else x0.clone()
         self._fresh = x0 is None
@@ -127,8 +165,11 @@ class GmresState:
         if self.phase == 'residual':
             if self._fresh and self.restarts == 0:
                 self.r.copy_(self.b)
+                self._res_prec = self.outer_prec
             else:
-                self.apply(self.x, self.r, self.outer_prec)
+                coarse = self.coarse_prec is not None and self._expect > self.coarse_above
+                self._res_prec = self.coarse_prec if coarse else self.outer_prec
+                self.apply(self.x, self.r, self._res_prec)
                 torch.sub(self.b, self.r, out=self.r)
             return torch.linalg.vector_norm(self.r).reshape(1)
         # arnoldi step j
@@ -164,6 +205,10 @@ class GmresState:
             if not math.isfinite(beta):
                 return self._finish('nonfinite')
             if beta <= self.target:
+                if self._res_prec != self.outer_prec:
+                    # a coarse residual never decides convergence
+                    self._expect = 0.0
+                    return False
                 return self._finish('converged')
             if self.it >= self.maxiter:
                 return self._finish('maxiter')
@@ -206,6 +251,7 @@ class GmresState:
             self.j = k
             return False
         # close the cycle: x += V_k y
+        self._expect = est / self.bnorm if math.isfinite(est) else float('inf')
         if math.isfinite(est):
             y = np.linalg.solve(np.triu(self.H[:k, :k]), self.g[:k])
             if np.all(np.isfinite(y)):
@@ -238,19 +284,23 @@ def gmres(apply, b, x0=None, tol=1e-8, restart=200, maxiter=2000, inner_prec='f6
 
 
 MODES = {
-    # name: (inner_prec, outer_prec, inner_reduction)
-    'f64': ('f64', 'f64', None),
-    'mixed': ('f32', 'f64', 1e-4),
+    # name: (inner_prec, outer_prec, inner_reduction, coarse residual prec)
+    'f64': ('f64', 'f64', None, None),
+    'mixed': ('f32', 'f64', 1e-4, 'f32'),
     # FP32 treecode farfield in the Arnoldi cycles (operators with a tree
-    # attached), exact FP64 direct farfield for every residual
-    'tree': ('tree', 'f64', 1e-4),
+    # attached), exact FP64 direct farfield for the residuals that decide;
+    # a cycle stops at 1e-3 since the treecode's own error (~1e-3) bounds
+    # what one cycle can gain
+    'tree': ('tree', 'f64', 1e-3, 'f32'),
 }
 
 
-def make_state(op, b, mode='mixed', tol=1e-8, restart=200, maxiter=2000, callback=None):
-    inner, outer, red = MODES[mode]
+def make_state(op, b, mode='mixed', tol=1e-8, restart=200, maxiter=2000, callback=None,
+               inner_reduction=None):
+    inner, outer, red, coarse = MODES[mode]
     return GmresState(op.apply, b, tol=tol, restart=restart, maxiter=maxiter, inner_prec=inner,
-                      outer_prec=outer, inner_reduction=red, callback=callback)
+                      outer_prec=outer, inner_reduction=inner_reduction or red, callback=callback,
+                      coarse_prec=coarse)
 
 
 def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxiter=6000,

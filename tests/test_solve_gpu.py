@@ -68,7 +68,7 @@ def test_tree_solve_residual_under_oracle_operator(oracle, order, mac):
     assert prob.H.farfield.tree.stats['m2p'] > 0
     u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=1e-8, mode='tree', restart=100)
     assert res.converged and res.rel_residual <= 1e-8
-    assert res.n_matvec['tree'] > res.n_matvec['f64'] > 0 and res.n_matvec['f32'] == 0
+    assert res.n_matvec['tree'] > res.n_matvec['f64'] > 0
     u = u.cpu().numpy()
     b = prob.cm.T @ prob.rhs.cpu().numpy()
     r = b - prob.cm.T @ oracle_apply(oracle, pts, tris, u)

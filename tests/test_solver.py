@@ -48,6 +48,33 @@ def test_mixed_precision_refinement_reaches_fp64_residual():
     assert np.linalg.norm(A @ res.x.numpy() - b.numpy()) <= 1e-12 * np.linalg.norm(b.numpy())
 
 
+def test_coarse_residuals_never_decide_convergence():
+    # three operators: 'tree' (inner, 1e-4 off), 'f32' (coarse residuals,
+    # 1e-7 off) and the exact 'f64'.  Early restarts may use the coarse
+    # operator; the last residual, which decides convergence, must be exact.
+    from tectosaur_amd.solver import GmresState
+    rng = np.random.default_rng(3)
+    n = 90
+    A = np.eye(n) * 3 + rng.normal(size=(n, n)) / np.sqrt(n)
+    ops = {'f64': torch.as_tensor(A),
+           'f32': torch.as_tensor(A + 1e-7 * rng.normal(size=(n, n)) / np.sqrt(n)),
+           'tree': torch.as_tensor(A + 1e-4 * rng.normal(size=(n, n)) / np.sqrt(n))}
+    used = []
+
+    def apply(x, out, prec):
+        used.append(prec)
+        torch.mv(ops[prec], x, out=out)
+        return out
+    b = torch.as_tensor(rng.normal(size=n))
+    st = GmresState(apply, b, tol=1e-10, restart=60, maxiter=2000, inner_prec='tree',
+                    outer_prec='f64', inner_reduction=1e-3, coarse_prec='f32', coarse_above=1e-4)
+    res = st.run()
+    assert res.converged
+    outer = [p for p in used if p != 'tree']
+    assert outer[-1] == 'f64' and 'f32' in outer
+    assert np.linalg.norm(A @ res.x.numpy() - b.numpy()) <= 1e-10 * np.linalg.norm(b.numpy())
+
+
 def test_gmres_zero_rhs_and_nan():
     A = np.eye(4)
     r = gmres(dense_apply(A), torch.zeros(4, dtype=torch.float64), tol=1e-8)
