@@ -165,6 +165,37 @@ def nearfield_bench(level=5, nq=8, thr=3.0, reps=20):
     return res
 
 
+def treecode_bench(pts, tris, v, settings=((3, 2.0), (4, 2.5))):
+    """The treecode farfield (FMMFarfieldOp) against the direct N-body on the
+    same workload, RH3 kernel (the served operator's farfield): time per
+    matvec and relative L2 error against the direct FP64 matvec."""
+    import torch
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    n = tris.shape[0]
+    idx = np.arange(n)
+    op = TriToTriDirectFarfieldOp(1, 'elasticRH3', PARAMS, pts, tris, np.float32, idx, idx,
+                                  precisions=('f32', 'f64'))
+    v_dev = torch.as_tensor(v, dtype=torch.float64, device='cuda')
+    y64 = op.dot_dev(v_dev, prec='f64')
+    nrm = torch.linalg.norm(y64).item()
+    res = {'kernel': 'elasticRH3', 'direct_f64_ms': 1e3 * stream_event_time(
+               lambda: op.dot_dev(v_dev, prec='f64'), 1),
+           'direct_f32_ms': 1e3 * stream_event_time(lambda: op.dot_dev(v_dev, prec='f32'), 2)}
+    for order, mac in settings:
+        t0 = time.perf_counter()
+        tr = op.attach_tree(order=order, mac=mac)
+        t_build = time.perf_counter() - t0
+        yt = op.dot_dev(v_dev, prec='tree')
+        err = (torch.linalg.norm(yt - y64) / nrm).item()
+        ms = 1e3 * stream_event_time(lambda: op.dot_dev(v_dev, prec='tree'), 3)
+        res['tree_o%d_mac%g' % (order, mac)] = {
+            'ms': ms, 'rel_l2_vs_f64_direct': err, 'speedup_vs_direct_f32': res['direct_f32_ms'] / ms,
+            'host_setup_s': t_build, 'm2p_nodes': tr.stats['m2p'], 'p2p_tiles': tr.stats['p2p_tiles'],
+            'segments': tr.n_seg}
+    print('# treecode: %s' % json.dumps(res), flush=True)
+    return res
+
+
 def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
     """The CPU oracle (C restatement of farfield_tris_to_tris, FP64 with
     long-double accumulation, OpenMP) on a bounded sample of observation
@@ -238,6 +269,7 @@ def main():
     ap.add_argument('--serve-rate', type=float, default=100.0)
     ap.add_argument('--no-serve', action='store_true')
     ap.add_argument('--no-nearfield', action='store_true')
+    ap.add_argument('--no-treecode', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run a 1 GiB device copy first (scale for byte counters)')
     args = ap.parse_args()
@@ -320,6 +352,8 @@ def main():
     }
     if calib is not None:
         line['copy_calibration'] = calib
+    if not args.no_treecode:
+        line['treecode'] = treecode_bench(pts, tris, v)
     if not args.no_nearfield:
         line['nearfield'] = nearfield_bench()
     if not args.no_serve:

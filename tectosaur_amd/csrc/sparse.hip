@@ -6,12 +6,13 @@
 // blocks are row-sorted (BSR) so every output row has one writer and no
 // atomics are needed.
 //
-// Layout: one 16-lane quarter-wave per block row.  Lane l < 9 owns output
-// row i = l of the 9x9 block row and walks all blocks of that block row,
-// reading the contiguous 9 doubles of row i of each block (72 B) and the 9
-// doubles of x it multiplies.  Four block rows per wavefront keep the loads
-// of neighbouring rows in flight together.  The kernel is HBM-bound: 648 B
-// of block data per block against 162 flops.
+// Layout: one wavefront per block row.  63 of its lanes are (b, i) pairs,
+// b < 7 a block of the current group of 7 consecutive blocks and i < 9 a
+// row of that block; a lane reads its block row (9 contiguous doubles) and
+// the 9 doubles of x it multiplies, so each step of the wave streams 7
+// whole blocks (4.5 KB contiguous).  The 7 partial sums per output row are
+// added in fixed order through LDS at the end.  HBM-bound: 648 B of block
+// data per block against 162 flops.
 #include "tct_common.h"
 
 namespace tct {
@@ -22,20 +23,32 @@ __global__ __launch_bounds__(256) void bsr9_kernel(int n_block_rows,
                                                    const double *__restrict__ data,
                                                    const double *__restrict__ x,
                                                    double *__restrict__ y, int accumulate) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int row = gid >> 4;
-    const int i = gid & 15;
-    if (row >= n_block_rows || i >= 9) return;
-    const int k0 = indptr[row], k1 = indptr[row + 1];
+    constexpr int NB = 7;
+    __shared__ double part[4][NB * 9];
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + wave;
+    const bool live = row < n_block_rows;
+    const int b = lane / 9, i = lane % 9;
     double s = 0.0;
-    for (int k = k0; k < k1; ++k) {
-        const double *A = data + (size_t)k * 81 + i * 9;
-        const double *xc = x + (size_t)indices[k] * 9;
+    if (live && lane < NB * 9) {
+        const int k0 = indptr[row], k1 = indptr[row + 1];
+        for (int k = k0 + b; k < k1; k += NB) {
+            const double *A = data + (size_t)k * 81 + i * 9;
+            const double *xc = x + (size_t)indices[k] * 9;
 #pragma unroll
-        for (int j = 0; j < 9; ++j) s += A[j] * xc[j];
+            for (int j = 0; j < 9; ++j) s += A[j] * xc[j];
+        }
     }
-    double *yr = y + (size_t)row * 9 + i;
-    *yr = accumulate ? *yr + s : s;
+    if (lane < NB * 9) part[wave][lane] = s;
+    __syncthreads();
+    if (live && lane < 9) {
+        double t = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < NB; ++bb) t += part[wave][bb * 9 + lane];
+        double *yr = y + (size_t)row * 9 + lane;
+        *yr = accumulate ? *yr + t : t;
+    }
 }
 
 __global__ __launch_bounds__(256) void csr_kernel(int n_rows, const int32_t *__restrict__ indptr,
@@ -60,8 +73,7 @@ extern "C" int tct_bsr9_spmv(int64_t n_block_rows, const int32_t *indptr,
     TCT_CHECK_ARG(n_block_rows >= 0 && n_block_rows < (1LL << 27), "bsr9: n_block_rows");
     TCT_CHECK_ARG(indptr && x && y, "bsr9: null pointer");
     if (n_block_rows == 0) return TCT_OK;
-    const int64_t threads = n_block_rows * 16;
-    hipLaunchKernelGGL(bsr9_kernel, dim3((unsigned)ceil_div(threads, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(bsr9_kernel, dim3((unsigned)ceil_div(n_block_rows, 4)), dim3(256), 0,
                        as_stream(stream), (int)n_block_rows, indptr, indices, data, x, y,
                        accumulate);
     TCT_CHECK_LAUNCH("bsr9_spmv");
