@@ -188,13 +188,6 @@ int tct_tree_p2m(int kernel, int prec, int order, int64_t n_src, int64_t n_tiles
 int tct_tree_m2m(int kernel, int prec, int order, int p_off, int n_par, int c_off, int n_child,
                  const double *node_cr, void *moments, void *stream);
 
-/* Up to four upward levels in one launch: levels l0 .. l0 + levels given by
- * off/cnt [levels + 1] (node offset and count per level, level l0 already
- * holding moments; each level halves the one below).  Needs
- * (2^(levels+1) - 1) * C * n_terms(order) values of LDS (<= 56 KiB). */
-int tct_tree_m2m_multi(int kernel, int prec, int order, int levels, const int *off,
-                       const int *cnt, const double *node_cr, void *moments, void *stream);
-
 /* Evaluation over interaction-list segments: list int32 entries e >= 0
  * (m2p from node e) or e < 0 (p2p with source tile -(e + 1)); seg
  * [n_seg][3] int32 = (observation tile, first entry, end entry), one
@@ -243,6 +236,15 @@ int tct_sub_f64(int64_t n, const double *a, const double *b, double *out, void *
 int tct_bsr9_spmv(int64_t n_block_rows, const int32_t *indptr, const int32_t *indices,
                   const double *data, const double *x, double *y, int accumulate,
                   void *stream);
+
+/* As tct_bsr9_spmv with the block data stored in FP32 (accumulation and
+ * vectors FP64): half the bytes, for operators used as an inner (inexact)
+ * operator of mixed-precision refinement.  No reference counterpart (the
+ * reference's float_type=float32 nearfield, tectosaur/util/sparse.py:29-33,
+ * is the same idea on the host). */
+int tct_bsr9_spmv_f32(int64_t n_block_rows, const int32_t *indptr, const int32_t *indices,
+                      const float *data, const double *x, double *y, int accumulate,
+                      void *stream);
 
 /* CSR SpMV y (+)= A x, FP64 (the constraint matrix cm and cm^T of
  * tectosaur/constraints.py:10-17 applied each GMRES iteration). */

@@ -54,7 +54,6 @@ SIGNATURES = {
     'tct_far_reduce_pts': [_i64, _i64, _int, _vp, _vp, _vp],
     'tct_tree_p2m': [_int, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     'tct_tree_m2m': [_int, _int, _int, _int, _int, _int, _int, _vp, _vp, _vp],
-    'tct_tree_m2m_multi': [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp],
     'tct_tree_eval': [_int, _int, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _vp,
                       _vp, _dbl, _vp, _vp],
     'tct_tree_reduce': [_int, _int, _int, _vp, _vp, _vp, _vp],
@@ -63,6 +62,7 @@ SIGNATURES = {
                             _vp, _vp],
     'tct_sub_f64': [_i64, _vp, _vp, _vp, _vp],
     'tct_bsr9_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
+    'tct_bsr9_spmv_f32': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_csr_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_find_near_pairs': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _vp, _i64, _vp],
     'tct_split_adjacent_close': [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -365,6 +365,8 @@ __global__ void scatter_tris_kernel(int64_t n_tris, int nq, const int32_t *__res
 
 // Galerkin scatter of the 9-moment kernels (RA, RH): applies the observation
 // triangle's surface curls.  tri_curl [n_tris][3][3] by DOF-space triangle.
+// One thread per triangle: each quadrature point's 9 moments are summed over
+// the chunks once and contracted with all three basis curls.
 template <int KID>
 __global__ void scatter_curl_kernel(int64_t n_tris, int nq, const int32_t *__restrict__ pt_of_tri,
                                     const double *__restrict__ qphi,
@@ -372,36 +374,50 @@ __global__ void scatter_curl_kernel(int64_t n_tris, int nq, const int32_t *__res
                                     const double *__restrict__ tri_curl, int64_t n_obs_pad,
                                     int chunks, const double *__restrict__ partial,
                                     ElasticConsts c, double *__restrict__ out) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n_tris * 9) return;
-    const int64_t t = gid / 9;
-    const int bk = (int)(gid % 9);
-    const int b = bk / 3, k = bk % 3;
-    const double *cu = tri_curl + 9 * t + 3 * b;  // curl of basis b (3 components)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tris) return;
+    double cu[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int I = 0; I < 3; ++I) cu[b][I] = tri_curl[9 * t + 3 * b + I];
     const int64_t p0 = (int64_t)pt_of_tri[t] * nq;
-    double s = 0.0;
+    double s[9];
+#pragma unroll
+    for (int o = 0; o < 9; ++o) s[o] = 0.0;
     for (int q = 0; q < nq; ++q) {
         const int64_t p = p0 + q;
         double M[9];
+#pragma unroll
         for (int a = 0; a < 9; ++a) {
             double r = 0.0;
             for (int ch = 0; ch < chunks; ++ch) r += partial[((size_t)ch * 9 + a) * n_obs_pad + p];
             M[a] = r;
         }
-        double val;
-        if constexpr (KID == TCT_RH) {
-            val = M[k * 3] * cu[0] + M[k * 3 + 1] * cu[1] + M[k * 3 + 2] * cu[2];
-        } else {
-            // sum_I curl[I] sum_s eps_{I s k} (CsRT1 v1_s + CsRT0 v2_s) - CsRT2 phi_b v3_k
-            double e = 0.0;
-            for (int I = 0; I < 3; ++I)
-                for (int ss = 0; ss < 3; ++ss)
-                    e += cu[I] * levi3(I, ss, k) * (c.CsRT1 * M[ss] + c.CsRT0 * M[3 + ss]);
-            val = e - c.CsRT2 * qphi[3 * q + b] * M[6 + k];
-        }
-        s += wj_obs[p] * val;
+        const double w = wj_obs[p];
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double val;
+                if constexpr (KID == TCT_RH) {
+                    val = M[k * 3] * cu[b][0] + M[k * 3 + 1] * cu[b][1] + M[k * 3 + 2] * cu[b][2];
+                } else {
+                    // sum_I curl[I] sum_s eps_{I s k} (CsRT1 v1_s + CsRT0 v2_s) - CsRT2 phi_b v3_k
+                    double e = 0.0;
+#pragma unroll
+                    for (int I = 0; I < 3; ++I)
+#pragma unroll
+                        for (int ss = 0; ss < 3; ++ss)
+                            e += cu[b][I] * levi3(I, ss, k) *
+                                 (c.CsRT1 * M[ss] + c.CsRT0 * M[3 + ss]);
+                    val = e - c.CsRT2 * qphi[3 * q + b] * M[6 + k];
+                }
+                s[b * 3 + k] += w * val;
+            }
     }
-    out[gid] = s;
+#pragma unroll
+    for (int o = 0; o < 9; ++o) out[9 * t + o] = s[o];
 }
 
 // ---------------------------------------------------------------------------
@@ -631,7 +647,7 @@ extern "C" int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const in
                   "scatter_curl: null pointer");
     if (n_tris == 0) return TCT_OK;
     const ElasticConsts c = elastic_consts(params_host);
-    const unsigned nb = (unsigned)ceil_div(n_tris * 9, 256);
+    const unsigned nb = (unsigned)ceil_div(n_tris, 256);
     if (kernel == TCT_RH)
         hipLaunchKernelGGL(scatter_curl_kernel<TCT_RH>, dim3(nb), dim3(256), 0,
                            as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl,

@@ -284,111 +284,6 @@ __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int
     }
 }
 
-// Several upward levels in one launch.  Levels l0 .. l0 + K (K <= 4) of the
-// tree are described by lv (node offset and count per level, index 0 =
-// level l0, whose moments exist).  Block r builds the subtree under node r
-// of level l0 + K: its 2^K bottom nodes are staged in LDS (dynamic shared
-// memory, (2^(K+1) - 1) nodes x C x n_terms(order) values of T), each level
-// is formed in LDS from the one below (same translation as
-// tree_m2m_kernel) and every new node is also written to `moments`.
-struct TreeLevels {
-    int off[5];
-    int cnt[5];
-};
-
-template <int KID, typename T>
-__global__ __launch_bounds__(256) void tree_m2m_multi_kernel(int order, int K, TreeLevels lv,
-                                                             const double *__restrict__ node_cr,
-                                                             T *__restrict__ moments) {
-    constexpr int C = TreeK<KID>::C;
-    constexpr int PM = TCT_TREE_MAX_ORDER + 1;
-    extern __shared__ unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
-    __shared__ double binom_t[PM][PM];
-    __shared__ double pw_d[16][3][PM];
-    __shared__ double pw_s[16][PM];
-    __shared__ double delta[16][3];
-    const int TT = n_terms(order);
-    const int NM = C * TT;
-    const int r = blockIdx.x;
-    const int tid = threadIdx.x;
-    if (tid < PM) {
-        double v = 1.0;  // row tid of Pascal's triangle
-        for (int kk = 0; kk < PM; ++kk) {
-            binom_t[tid][kk] = kk <= tid ? v : 0.0;
-            v = v * (tid - kk) / (kk + 1);
-        }
-    }
-    // LDS slot of level s: nodes of levels < s come first
-    auto slot = [&](int s) {
-        int b = 0;
-        for (int u = 0; u < s; ++u) b += 1 << (K - u);
-        return b;
-    };
-    {
-        const int n0 = r << K;
-        const int nb = min(1 << K, lv.cnt[0] - n0);
-        for (int o = tid; o < nb * NM; o += blockDim.x)
-            lds[o] = moments[(size_t)(lv.off[0] + n0 + o / NM) * NM + o % NM];
-    }
-    __syncthreads();
-    for (int s = 1; s <= K; ++s) {
-        const int np0 = r << (K - s), nc0 = r << (K - s + 1);
-        const int np = min(1 << (K - s), lv.cnt[s] - np0);
-        const int ncn = min(1 << (K - s + 1), lv.cnt[s - 1] - nc0);
-        T *Lc = lds + (size_t)slot(s - 1) * NM;
-        T *Lp = lds + (size_t)slot(s) * NM;
-        if (tid < ncn * 4) {
-            const int jj = tid / 4, k = tid % 4;
-            const double *cp = node_cr + 4 * (size_t)(lv.off[s] + np0 + jj / 2);
-            const double *cc = node_cr + 4 * (size_t)(lv.off[s - 1] + nc0 + jj);
-            const double base = k < 3 ? (cc[k] - cp[k]) / cp[3] : cc[3] / cp[3];
-            double v = 1.0;
-            for (int e = 0; e < PM; ++e) {
-                if (k < 3)
-                    pw_d[jj][k][e] = v;
-                else
-                    pw_s[jj][e] = v;
-                v *= base;
-            }
-            if (k < 3) delta[jj][k] = cc[k] - cp[k];
-        }
-        __syncthreads();
-        // the children's Psi channels absorb the shift to their parent's centre
-        for (int o = tid; o < ncn * psi_count<KID>() * TT; o += blockDim.x) {
-            const int jj = o / (psi_count<KID>() * TT), rr = o % (psi_count<KID>() * TT);
-            const int ps = rr / TT, bi = rr % TT;
-            T *M = Lc + (size_t)jj * NM;
-            double add = 0.0;
-            for (int b = 0; b < 3; ++b)
-                add += delta[jj][b] * (double)M[psi_source<KID>(ps, b) * TT + bi];
-            M[psi_channel<KID>(ps) * TT + bi] = (T)((double)M[psi_channel<KID>(ps) * TT + bi] + add);
-        }
-        __syncthreads();
-        for (int o = tid; o < np * NM; o += blockDim.x) {
-            const int jj = o / NM, rr = o % NM;
-            const int ch = rr / TT;
-            const Mi m = mi_at(rr % TT);
-            double acc = 0.0;
-            for (int cc = 2 * jj; cc < 2 * jj + 2 && cc < ncn; ++cc) {
-                const T *M = Lc + (size_t)cc * NM + ch * TT;
-                for (int ba = 0; ba <= m.a; ++ba) {
-                    const double ca = binom_t[m.a][ba] * pw_d[cc][0][m.a - ba];
-                    for (int bb = 0; bb <= m.b; ++bb) {
-                        const double cb = ca * binom_t[m.b][bb] * pw_d[cc][1][m.b - bb];
-                        for (int bc = 0; bc <= m.c; ++bc)
-                            acc += cb * binom_t[m.c][bc] * pw_d[cc][2][m.c - bc] *
-                                   pw_s[cc][ba + bb + bc] * (double)M[mi_pos(ba, bb, bc)];
-                    }
-                }
-            }
-            Lp[(size_t)jj * NM + rr] = (T)acc;
-            moments[(size_t)(lv.off[s] + np0 + jj) * NM + rr] = (T)acc;
-        }
-        __syncthreads();
-    }
-}
-
 // ---------------------------------------------------------------------------
 // One multipole node evaluated at one observation point.  a: derivatives up
 // to order P + 1 at u = (c - x)/rho; Rt = c - x (physical); M: the node's

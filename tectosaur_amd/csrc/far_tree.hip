@@ -90,49 +90,6 @@ extern "C" int tct_tree_m2m(int kernel, int prec, int order, int p_off, int n_pa
     return TCT_OK;
 }
 
-extern "C" int tct_tree_m2m_multi(int kernel, int prec, int order, int levels, const int *off,
-                                  const int *cnt, const double *node_cr, void *moments,
-                                  void *stream) {
-    TCT_CHECK_ARG(tree_kernel_ok(kernel), "tree_m2m_multi: kernel %d has no treecode form",
-                  kernel);
-    TCT_CHECK_ARG(prec == TCT_F32 || prec == TCT_F64, "tree_m2m_multi: precision %d", prec);
-    TCT_CHECK_ARG(order >= 0 && order <= TCT_TREE_MAX_ORDER, "tree_m2m_multi: order %d", order);
-    TCT_CHECK_ARG(levels >= 1 && levels <= 4, "tree_m2m_multi: levels %d (1..4)", levels);
-    TCT_CHECK_ARG(off && cnt && node_cr && moments, "tree_m2m_multi: null pointer");
-    TreeLevels lv{};
-    for (int s = 0; s <= levels; ++s) {
-        lv.off[s] = off[s];
-        lv.cnt[s] = cnt[s];
-        TCT_CHECK_ARG(cnt[s] > 0 && off[s] >= 0, "tree_m2m_multi: level %d empty", s);
-        if (s > 0)
-            TCT_CHECK_ARG(cnt[s] == (cnt[s - 1] + 1) / 2 && off[s] >= off[s - 1] + cnt[s - 1],
-                          "tree_m2m_multi: level %d does not halve level %d", s, s - 1);
-    }
-    const int C = kernel == TCT_RT ? 13 : (kernel == TCT_RH ? 12 : 4);
-    const size_t tsz = prec == TCT_F32 ? sizeof(float) : sizeof(double);
-    const size_t lds = (size_t)((2 << levels) - 1) * C * n_terms(order) * tsz;
-    TCT_CHECK_ARG(lds <= 56 * 1024, "tree_m2m_multi: %d levels of order %d need %zu B of LDS",
-                  levels, order, lds);
-    hipStream_t st = as_stream(stream);
-    const dim3 grid((unsigned)cnt[levels]), block(256);
-#define TCT_MM(KID)                                                                          \
-    if (prec == TCT_F32)                                                                     \
-        hipLaunchKernelGGL((tree_m2m_multi_kernel<KID, float>), grid, block, lds, st, order, \
-                           levels, lv, node_cr, (float *)moments);                           \
-    else                                                                                     \
-        hipLaunchKernelGGL((tree_m2m_multi_kernel<KID, double>), grid, block, lds, st,       \
-                           order, levels, lv, node_cr, (double *)moments)
-    switch (kernel) {
-    case TCT_U: TCT_MM(TCT_U); break;
-    case TCT_RT: TCT_MM(TCT_RT); break;
-    case TCT_RA: TCT_MM(TCT_RA); break;
-    default: TCT_MM(TCT_RH); break;
-    }
-#undef TCT_MM
-    TCT_CHECK_LAUNCH("tree_m2m_multi");
-    return TCT_OK;
-}
-
 extern "C" int tct_tree_eval(int kernel, int prec, int order, int n_obs_pad, const void *obs,
                              const double *obs_org, const void *rec, const double *src_org,
                              const double *node_cr, const void *moments, int n_seg,

@@ -17,10 +17,11 @@
 
 namespace tct {
 
+template <typename TD>
 __global__ __launch_bounds__(256) void bsr9_kernel(int n_block_rows,
                                                    const int32_t *__restrict__ indptr,
                                                    const int32_t *__restrict__ indices,
-                                                   const double *__restrict__ data,
+                                                   const TD *__restrict__ data,
                                                    const double *__restrict__ x,
                                                    double *__restrict__ y, int accumulate) {
     constexpr int NB = 7;
@@ -34,10 +35,10 @@ __global__ __launch_bounds__(256) void bsr9_kernel(int n_block_rows,
     if (live && lane < NB * 9) {
         const int k0 = indptr[row], k1 = indptr[row + 1];
         for (int k = k0 + b; k < k1; k += NB) {
-            const double *A = data + (size_t)k * 81 + i * 9;
+            const TD *A = data + (size_t)k * 81 + i * 9;
             const double *xc = x + (size_t)indices[k] * 9;
 #pragma unroll
-            for (int j = 0; j < 9; ++j) s += A[j] * xc[j];
+            for (int j = 0; j < 9; ++j) s += (double)A[j] * xc[j];
         }
     }
     if (lane < NB * 9) part[wave][lane] = s;
@@ -73,10 +74,23 @@ extern "C" int tct_bsr9_spmv(int64_t n_block_rows, const int32_t *indptr,
     TCT_CHECK_ARG(n_block_rows >= 0 && n_block_rows < (1LL << 27), "bsr9: n_block_rows");
     TCT_CHECK_ARG(indptr && x && y, "bsr9: null pointer");
     if (n_block_rows == 0) return TCT_OK;
-    hipLaunchKernelGGL(bsr9_kernel, dim3((unsigned)ceil_div(n_block_rows, 4)), dim3(256), 0,
-                       as_stream(stream), (int)n_block_rows, indptr, indices, data, x, y,
+    hipLaunchKernelGGL(bsr9_kernel<double>, dim3((unsigned)ceil_div(n_block_rows, 4)), dim3(256),
+                       0, as_stream(stream), (int)n_block_rows, indptr, indices, data, x, y,
                        accumulate);
     TCT_CHECK_LAUNCH("bsr9_spmv");
+    return TCT_OK;
+}
+
+extern "C" int tct_bsr9_spmv_f32(int64_t n_block_rows, const int32_t *indptr,
+                                 const int32_t *indices, const float *data, const double *x,
+                                 double *y, int accumulate, void *stream) {
+    TCT_CHECK_ARG(n_block_rows >= 0 && n_block_rows < (1LL << 27), "bsr9_f32: n_block_rows");
+    TCT_CHECK_ARG(indptr && x && y, "bsr9_f32: null pointer");
+    if (n_block_rows == 0) return TCT_OK;
+    hipLaunchKernelGGL(bsr9_kernel<float>, dim3((unsigned)ceil_div(n_block_rows, 4)), dim3(256),
+                       0, as_stream(stream), (int)n_block_rows, indptr, indices, data, x, y,
+                       accumulate);
+    TCT_CHECK_LAUNCH("bsr9_spmv_f32");
     return TCT_OK;
 }
 

@@ -17,7 +17,7 @@ rest for the GPU (far_tree.h):
     (m2p), the rest refined down to source tiles evaluated directly (p2p);
   * multipoles are Cartesian Taylor moments of 1/r with per-kernel Laplace
     charges; moments of parents come from children (m2m), one launch per
-    level.
+    level up to the highest level the lists use.
 
 ``pts_per_cell`` sets the smallest source node evaluated directly: ranges of
 ``max(64, pts_per_cell)`` points rounded up to a power-of-two number of
@@ -222,11 +222,11 @@ class TreeFarfield:
         self.seg = torch.as_tensor(seg, device=dev)
         self.seg_off = torch.as_tensor(seg_off, device=dev)
         self.chunks = 1  # the reduced output has one chunk
-        # upward pass: only up to the highest level any list refers to, in
-        # launches of up to 4 levels (as many as fit the LDS budget)
+        # upward pass: one launch per level, only up to the highest level any
+        # list refers to (a fused multi-level kernel measured slower: its
+        # deep per-block chains leave the GPU idle)
         used = lst[lst >= 0]
         self.max_level = int(self.tree.level[used].max()) if used.size else 0
-        self._m2m_groups = {}
         self._moments = {}
         self._partial = None
         self._seg_partial = None
@@ -237,26 +237,6 @@ class TreeFarfield:
             self._moments[prec] = torch.empty(self.tree.n_nodes * self.C * self.T, dtype=dt,
                                               device=self.engine.device)
         return self._moments[prec]
-
-    def m2m_groups(self, prec, lds_budget=56 * 1024):
-        """(levels, off, cnt) per upward launch for this precision."""
-        if prec not in self._m2m_groups:
-            tsz = 4 if prec == 'f32' else 8
-            node = self.C * self.T * tsz
-            kmax = max(k for k in range(1, 5) if ((2 << k) - 1) * node <= lds_budget) \
-                if 3 * node <= lds_budget else 0
-            if kmax == 0:
-                raise N.TctError('treecode order %d too large for the upward pass' % self.order)
-            groups = []
-            l0 = 0
-            while l0 < self.max_level:
-                k = min(kmax, self.max_level - l0)
-                lv = self.tree.levels[l0:l0 + k + 1]
-                groups.append((k, np.array([o for o, _ in lv], dtype=np.int32),
-                               np.array([c for _, c in lv], dtype=np.int32)))
-                l0 += k
-            self._m2m_groups[prec] = groups
-        return self._m2m_groups[prec]
 
     @property
     def partial(self):
@@ -277,9 +257,11 @@ class TreeFarfield:
         rec = e.rec(prec)
         N.call('tct_tree_p2m', e.kid, pid, self.order, e.n_src, self.tree.n_tiles, N.ptr(rec),
                N.ptr(e.src_org), N.ptr(self.node_cr), N.ptr(mom), st)
-        for levels, off, cnt in self.m2m_groups(prec):
-            N.call('tct_tree_m2m_multi', e.kid, pid, self.order, levels, off.ctypes.data,
-                   cnt.ctypes.data, N.ptr(self.node_cr), N.ptr(mom), st)
+        for lvl in range(1, self.max_level + 1):
+            p_off, n_par = self.tree.levels[lvl]
+            c_off, n_child = self.tree.levels[lvl - 1]
+            N.call('tct_tree_m2m', e.kid, pid, self.order, p_off, n_par, c_off, n_child,
+                   N.ptr(self.node_cr), N.ptr(mom), st)
         obs = e.loc32 if prec == 'f32' else e.obs_soa64
         out = self.partial
         N.call('tct_tree_eval', e.kid, pid, self.order, e.n_obs_pad, N.ptr(obs), N.ptr(e.obs_org),

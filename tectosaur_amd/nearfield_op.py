@@ -161,8 +161,16 @@ class RegularizedNearfieldIntegralOp:
             self.mat_no_correction = self.mat
         self.n_pairs = sum(self.counts.values())
 
-    def dot_dev(self, x, out=None, accumulate=False):
-        return self.mat.dot_dev(x, out, accumulate)
+    def dot_dev(self, x, out=None, accumulate=False, prec='f64'):
+        """prec other than 'f64' uses the FP32 copy of the blocks when one
+        exists (``use_f32_inner``)."""
+        return self.mat.dot_dev(x, out, accumulate, prec)
+
+    def use_f32_inner(self):
+        """Keep an FP32 copy of the corrected blocks for inexact (inner)
+        matvecs of mixed-precision solves; FP64 products are unchanged."""
+        self.mat.make_f32()
+        return self
 
     def dot(self, v):
         return self.mat.dot(v)

@@ -64,16 +64,18 @@ class SolveRecord:
                     n_matvec=self.n_matvec)
 
 
-def estimate_bytes(n_tris, restart):
+def estimate_bytes(n_tris, restart, tree=False):
     """Device footprint of one solve: farfield layouts and partial sums,
-    nearfield blocks (corrected and uncorrected, ~45 pairs per triangle),
-    constraint and mass matrices, Krylov basis."""
+    nearfield blocks (FP64 and the FP32 inner copy, ~90 pairs per triangle
+    at the default threshold), constraint and mass matrices, Krylov basis,
+    and the treecode's moments and segment partials when it has one."""
     n_pts = 3 * n_tris
     far = n_pts * (24 + 48 + 96 + 48 + 8 * 3 * 8 + 16)
-    near = 45 * n_tris * (2 * 648 + 16)
+    near = 90 * n_tris * (648 + 324 + 16)
     n_free = int(1.6 * n_tris) + 64
     krylov = (restart + 5) * n_free * 8
-    return far + near + krylov + 64 * n_tris * 8
+    tr = 80e6 + n_pts * 64 if tree else 0
+    return int(far + near + krylov + 64 * n_tris * 8 + tr)
 
 
 class _Active:
@@ -170,7 +172,9 @@ class Server:
                     req = pending[qi]
                     if realtime and req.arrival > now:
                         break
-                    need = estimate_bytes(req.n_tris, self.restart)
+                    need = estimate_bytes(req.n_tris, self.restart,
+                                          self.tree is not None and
+                                          req.n_tris >= self.tree.get('min_tris', 0))
                     if active and used + need > budget:
                         break
                     if pool is not None:

@@ -55,20 +55,30 @@ class ConstrainedOperator:
         precision in ``precs`` into a HIP graph replayed by ``apply``: one
         graph launch and two vector copies per application instead of a
         dozen kernel launches.  The captured kernels write every output in
-        full (no memsets) and use only buffers the operator owns."""
+        full (no memsets) and use only buffers the operator owns.
+
+        Capture runs on a side stream ordered after the current one, without
+        a device-wide synchronisation, so a server can build a new solve's
+        graphs while other solves' work is in flight."""
         n = self.shape[0]
         self._gx = torch.zeros(n, dtype=torch.float64, device=self.device)
         self._gy = torch.empty(n, dtype=torch.float64, device=self.device)
+        cur = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(self.device)
         for prec in precs:
             if prec is None or prec in self._graphs:
                 continue
             self._apply_raw(self._gx, self._gy, prec)      # allocate lazy buffers eagerly
-            torch.cuda.synchronize(self.device)
+            side.wait_stream(cur)
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                self._apply_raw(self._gx, self._gy, prec)
+            with torch.cuda.stream(side):
+                g.capture_begin()
+                try:
+                    self._apply_raw(self._gx, self._gy, prec)
+                finally:
+                    g.capture_end()
+            cur.wait_stream(side)
             self._graphs[prec] = g
-        torch.cuda.synchronize(self.device)
 
     def apply(self, x, out=None, prec='f64'):
         if out is None:

@@ -24,6 +24,7 @@ class BSRMatrix:
         self.indices = torch.as_tensor(indices, device=dev).to(torch.int32).contiguous()
         d = torch.as_tensor(data, device=dev).to(torch.float64)
         self.data = d.reshape((-1, 81)).contiguous()
+        self.data32 = None
         self.shape = tuple(shape)
         if self.shape[0] % 9 or self.shape[1] % 9:
             raise ValueError('shape must be a multiple of the 9x9 block size')
@@ -38,12 +39,26 @@ class BSRMatrix:
     def nnzb(self):
         return self.indices.shape[0]
 
-    def dot_dev(self, x, out=None, accumulate=False):
+    def make_f32(self):
+        """Keep an FP32 copy of the block data for inexact (inner) products."""
+        if self.data32 is None:
+            self.data32 = self.data.to(torch.float32)
+        return self
+
+    def dot_dev(self, x, out=None, accumulate=False, prec='f64'):
+        """y (+)= A x.  prec 'f64' uses the FP64 blocks; any other precision
+        uses the FP32 copy when one was made (FP64 accumulation either way)."""
         if out is None:
             out = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
             accumulate = False
-        N.call('tct_bsr9_spmv', self.shape[0] // 9, N.ptr(self.indptr), N.ptr(self.indices),
-               N.ptr(self.data), N.ptr(x), N.ptr(out), int(bool(accumulate)), N.stream_ptr())
+        if prec != 'f64' and self.data32 is not None:
+            N.call('tct_bsr9_spmv_f32', self.shape[0] // 9, N.ptr(self.indptr),
+                   N.ptr(self.indices), N.ptr(self.data32), N.ptr(x), N.ptr(out),
+                   int(bool(accumulate)), N.stream_ptr())
+        else:
+            N.call('tct_bsr9_spmv', self.shape[0] // 9, N.ptr(self.indptr), N.ptr(self.indices),
+                   N.ptr(self.data), N.ptr(x), N.ptr(out), int(bool(accumulate)),
+                   N.stream_ptr())
         return out
 
     def dot(self, v):

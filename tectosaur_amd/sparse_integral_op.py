@@ -44,9 +44,10 @@ class RegularizedSparseIntegralOp:
 
     def dot_dev(self, v_dev, out=None, prec=None):
         """Device matvec in FP64 storage; ``prec`` selects the farfield
-        evaluation precision ('f32' or 'f64')."""
+        evaluation ('f32', 'f64' or 'tree') and, when the nearfield keeps an
+        FP32 copy, which nearfield blocks are read (FP64 for 'f64')."""
         out = self.farfield.dot_dev(v_dev, out, prec)
-        self.nearfield.dot_dev(v_dev, out, accumulate=True)
+        self.nearfield.dot_dev(v_dev, out, accumulate=True, prec=prec or 'f64')
         return out
 
     def dot(self, v):

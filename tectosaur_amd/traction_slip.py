@@ -171,6 +171,9 @@ class TractionSlipProblem:
             near_plan=prepared['near_plan'], far_plan=prepared['far_plan'],
             keep_uncorrected=False)
         self.device = self.H.device
+        if 'f32' in farfield_precisions:
+            # inner (FP32 / treecode) matvecs read FP32 nearfield blocks
+            self.H.nearfield.use_f32_inner()
         if prepared.get('tree') is not None:
             tr = prepared['tree']
             self.H.farfield.attach_tree(order=tr['order'], mac=tr['mac'], plan=tr['plan'])
