@@ -136,6 +136,9 @@ class Server:
         self.prep_workers = prep_workers
         # capture each solve's operator applications into HIP graphs
         self.graphs = graphs
+        # batched servers build new solves on a stream of their own
+        # (default priority) so admissions overlap the running solves
+        self.build_stream = torch.cuda.Stream() if max_active > 1 else None
         self.ticks = 0
 
     def run(self, requests, realtime=True, log=None):
@@ -192,8 +195,17 @@ class Server:
                     if prep is None:
                         prep = _prepare(req, self.orders, self.tree)
                     rec.prep_s = prep['prep_s']
-                    prob, op, state = _build(prep, self.mode, self.tol, self.restart,
-                                             self.maxiter, self.graphs)
+                    if self.build_stream is not None:
+                        # build on its own stream: its host copies and syncs
+                        # then wait only for the build, not for the ticks
+                        # already queued on the compute stream
+                        with torch.cuda.stream(self.build_stream):
+                            prob, op, state = _build(prep, self.mode, self.tol, self.restart,
+                                                     self.maxiter, self.graphs)
+                        torch.cuda.current_stream().wait_stream(self.build_stream)
+                    else:
+                        prob, op, state = _build(prep, self.mode, self.tol, self.restart,
+                                                 self.maxiter, self.graphs)
                     rec.n_tris = prob.n_tris
                     rec.pair_evals = prob.pair_evals
                     rec.setup_s = clock() - rec.admitted

@@ -121,7 +121,7 @@ def fault_mesh(req):
 # treecode of the served 'tree' mode: expansion order, acceptance parameter
 # and the smallest problem (triangles) that gets one -- below it the direct
 # FP32 N-body is faster than the treecode's fixed per-matvec passes
-DEFAULT_TREE = dict(order=3, mac=2.0, min_tris=8192)
+DEFAULT_TREE = dict(order=3, mac=2.0, min_tris=3000)
 
 
 def prepare_problem(pts, tris, traction, sm=3e10, orders=None, K_far='elasticRH3', tree=None):
