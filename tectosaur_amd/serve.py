@@ -4,12 +4,15 @@ Requests (``traction_slip.FaultRequest``) arrive over time.  The loop
 admits an arrived request as soon as a slot is free and its estimated
 device footprint fits the free-HBM budget, builds its operators on the
 device, and from then on keeps its whole Krylov state in HBM.  Every tick
-advances every active solve by one GMRES step: each solve enqueues its
-device work (matvec, orthogonalisation) on the stream, the small per-solve
-results (Hessenberg column or residual norm) are concatenated on the device
-and cross to the host in one transfer, and the host-side Givens updates run.
-A converged (or failed) solve leaves the batch at once and its slot and
-memory are reused; requests arriving meanwhile join the next tick.
+advances a batch of active solves by one GMRES step: each solve enqueues
+its device work (matvec as a HIP-graph replay, orthogonalisation) on the
+stream, the small per-solve results (Hessenberg column or residual norm)
+are concatenated on the device and cross to the host in one asynchronous
+transfer, and the host-side Givens updates run.  The active solves are
+split into two batches whose ticks alternate, so the device works on one
+batch while the host consumes the other's results.  A converged (or
+failed) solve leaves at once and its slot and memory are reused; requests
+arriving meanwhile join the next batch issued.
 
 Host-side set-up of arriving requests (mesh, pair classification, farfield
 layout, mass and constraint matrices) runs on a pool of worker threads while
@@ -23,6 +26,7 @@ and in the mix.
 ``serve_sequential`` is the one-at-a-time reference path (plain FP64 by
 default) the batched path is compared with.
 """
+import collections
 import concurrent.futures
 import time
 
@@ -85,6 +89,7 @@ class _Active:
         self.op = op
         self.state = state
         self.nbytes = nbytes
+        self.busy = False     # a step is in flight
 
 
 def _prepare(req, orders, tree=None):
@@ -141,6 +146,37 @@ class Server:
         self.build_stream = torch.cuda.Stream() if max_active > 1 else None
         self.ticks = 0
 
+    def _build_on_stream(self, prep):
+        """Device build of one solve on the build stream (builder thread);
+        returns its objects and an event the compute stream waits on."""
+        with torch.cuda.stream(self.build_stream):
+            prob, op, state = _build(prep, self.mode, self.tol, self.restart, self.maxiter,
+                                     self.graphs)
+            ev = torch.cuda.Event()
+            ev.record()
+        return prob, op, state, ev
+
+    @staticmethod
+    def _adopt(active, rec, prob, op, state, need, clock):
+        rec.n_tris = prob.n_tris
+        rec.pair_evals = prob.pair_evals
+        rec.setup_s = clock() - rec.admitted
+        active.append(_Active(rec, prob, op, state, need))
+
+    def _issue(self, batch):
+        """Enqueue one step of every solve in ``batch``; their small results
+        are gathered on the device and copied to pinned host memory without
+        blocking.  Returns (batch, sizes, pinned buffer, event)."""
+        outs = [a.state.issue() for a in batch]
+        for a in batch:
+            a.busy = True
+        dev = torch.cat(outs)
+        pinned = torch.empty(dev.shape, dtype=dev.dtype, pin_memory=True)
+        pinned.copy_(dev, non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        return batch, [o.shape[0] for o in outs], pinned, ev
+
     def run(self, requests, realtime=True, log=None):
         """Serve ``requests`` (sorted by arrival).  With realtime=True a
         request is admitted no earlier than its arrival time after start;
@@ -160,8 +196,12 @@ class Server:
         clock = lambda: time.perf_counter() - t0
         qi = 0    # next request to admit
         si = 0    # next request to hand to the preparation pool
+        inflight = collections.deque()
+        building = collections.deque()
+        builder = (concurrent.futures.ThreadPoolExecutor(1) if self.build_stream is not None
+                   else None)
         try:
-            while qi < len(pending) or active:
+            while qi < len(pending) or active or building:
                 now = clock()
                 # host preparation of arrived requests, bounded look-ahead
                 while pool is not None and si < len(pending) and si < qi + max_ahead:
@@ -171,18 +211,19 @@ class Server:
                                                            self.tree)
                     si += 1
                 # admission (FIFO) of prepared requests
-                while qi < len(pending) and len(active) < self.max_active:
+                while (qi < len(pending) and
+                       len(active) + len(building) < self.max_active):
                     req = pending[qi]
                     if realtime and req.arrival > now:
                         break
                     need = estimate_bytes(req.n_tris, self.restart,
                                           self.tree is not None and
                                           req.n_tris >= self.tree.get('min_tris', 0))
-                    if active and used + need > budget:
+                    if (active or building) and used + need > budget:
                         break
                     if pool is not None:
                         fut = futures.get(req.rid)
-                        if fut is None or (active and not fut.done()):
+                        if fut is None or ((active or building) and not fut.done()):
                             break
                         prep = fut.result()
                         del futures[req.rid]
@@ -195,51 +236,67 @@ class Server:
                     if prep is None:
                         prep = _prepare(req, self.orders, self.tree)
                     rec.prep_s = prep['prep_s']
-                    if self.build_stream is not None:
-                        # build on its own stream: its host copies and syncs
-                        # then wait only for the build, not for the ticks
-                        # already queued on the compute stream
-                        with torch.cuda.stream(self.build_stream):
-                            prob, op, state = _build(prep, self.mode, self.tol, self.restart,
-                                                     self.maxiter, self.graphs)
-                        torch.cuda.current_stream().wait_stream(self.build_stream)
-                    else:
-                        prob, op, state = _build(prep, self.mode, self.tol, self.restart,
-                                                 self.maxiter, self.graphs)
-                    rec.n_tris = prob.n_tris
-                    rec.pair_evals = prob.pair_evals
-                    rec.setup_s = clock() - rec.admitted
-                    active.append(_Active(rec, prob, op, state, need))
                     recs[req.rid] = rec
                     used += need
                     qi += 1
+                    if builder is not None:
+                        # the device build runs on the builder thread and
+                        # the build stream while this thread keeps ticking
+                        building.append((builder.submit(self._build_on_stream, prep), rec,
+                                         need))
+                    else:
+                        prob, op, state = _build(prep, self.mode, self.tol, self.restart,
+                                                 self.maxiter, self.graphs)
+                        self._adopt(active, rec, prob, op, state, need, clock)
                     now = clock()
+                # adopt finished builds (block only when nothing else can run)
+                while building and (building[0][0].done() or not active):
+                    fut, rec, need = building.popleft()
+                    prob, op, state, ev = fut.result()
+                    torch.cuda.current_stream().wait_event(ev)
+                    self._adopt(active, rec, prob, op, state, need, clock)
                 if not active:
                     if qi < len(pending):
                         time.sleep(max(0.0, min(0.002, pending[qi].arrival - clock())))
                     continue
-                # one tick: every active solve advances one step, one transfer
-                outs = [a.state.issue() for a in active]
-                sizes = [o.shape[0] for o in outs]
-                host = torch.cat(outs).cpu().numpy()
+                # Ticks are pipelined over (up to) two batches of solves: a
+                # batch's step is issued, its small results head to pinned
+                # host memory asynchronously, and while the host consumes one
+                # batch (Givens updates, convergence) the device runs the
+                # other batch's queued step.
+                ready = [a for a in active if not a.busy]
+                if ready and len(inflight) < 2:
+                    if not inflight and len(ready) > 1:
+                        half = (len(ready) + 1) // 2
+                        inflight.append(self._issue(ready[:half]))
+                        inflight.append(self._issue(ready[half:]))
+                    else:
+                        inflight.append(self._issue(ready))
+                    continue
+                members, sizes, pinned, ev = inflight.popleft()
+                ev.synchronize()
+                host = pinned.numpy()
                 self.ticks += 1
                 off = 0
-                still = []
                 now = clock()
-                for a, sz in zip(active, sizes):
+                finished = set()
+                for a, sz in zip(members, sizes):
                     done = a.state.consume(host[off:off + sz])
                     off += sz
+                    a.busy = False
                     if done:
                         _finalize(a, now, self.keep_slip)
                         used -= a.nbytes
+                        finished.add(id(a))
                         if log:
                             log(a.rec)
-                    else:
-                        still.append(a)
-                active = still
+                if finished:
+                    active = [a for a in active if id(a) not in finished]
         finally:
             if pool is not None:
                 pool.shutdown(wait=True, cancel_futures=True)
+            if builder is not None:
+                builder.shutdown(wait=True, cancel_futures=True)
         torch.cuda.synchronize()
         self.makespan = clock()
         return [recs[r.rid] for r in pending]

@@ -72,7 +72,9 @@ class ConstrainedOperator:
             side.wait_stream(cur)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.stream(side):
-                g.capture_begin()
+                # thread-local: other host threads may keep launching work
+                # on other streams while this one captures
+                g.capture_begin(capture_error_mode='thread_local')
                 try:
                     self._apply_raw(self._gx, self._gy, prec)
                 finally:
