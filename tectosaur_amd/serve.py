@@ -92,6 +92,14 @@ class _Active:
         self.busy = False     # a step is in flight
 
 
+def _release(a):
+    """Drop a finished solve's device objects (and with them its HIP graphs)
+    on the calling thread."""
+    a.op = None
+    a.state = None
+    a.prob = None
+
+
 def _prepare(req, orders, tree=None):
     t0 = time.perf_counter()
     pts, tris = fault_mesh(req)
@@ -171,9 +179,16 @@ class Server:
         for a in batch:
             a.busy = True
         dev = torch.cat(outs)
-        pinned = torch.empty(dev.shape, dtype=dev.dtype, pin_memory=True)
+        # one of two pinned buffers and events allocated up front: no host
+        # allocation (an unsafe call during another thread's graph capture)
+        # happens in the loop
+        k = self._slot
+        self._slot ^= 1
+        if self._pinned[k].shape[0] < dev.shape[0]:
+            raise RuntimeError('batch results exceed the pinned buffer')
+        pinned = self._pinned[k][:dev.shape[0]]
         pinned.copy_(dev, non_blocking=True)
-        ev = torch.cuda.Event()
+        ev = self._events[k]
         ev.record()
         return batch, [o.shape[0] for o in outs], pinned, ev
 
@@ -198,6 +213,11 @@ class Server:
         si = 0    # next request to hand to the preparation pool
         inflight = collections.deque()
         building = collections.deque()
+        # per batch at most max_active solves x (restart + 2) values
+        cap = self.max_active * (self.restart + 2)
+        self._pinned = [torch.empty(cap, dtype=torch.float64, pin_memory=True) for _ in range(2)]
+        self._events = [torch.cuda.Event() for _ in range(2)]
+        self._slot = 0
         builder = (concurrent.futures.ThreadPoolExecutor(1) if self.build_stream is not None
                    else None)
         try:
@@ -290,6 +310,10 @@ class Server:
                         finished.add(id(a))
                         if log:
                             log(a.rec)
+                        if builder is not None:
+                            # the solve's graphs are destroyed on the builder
+                            # thread, never while that thread is capturing
+                            builder.submit(_release, a)
                 if finished:
                     active = [a for a in active if id(a) not in finished]
         finally:
