@@ -228,6 +228,48 @@ int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, const double *
  * nearfield/nearfield_op.py:135-141). */
 int tct_sub_f64(int64_t n, const double *a, const double *b, double *out, void *stream);
 
+/* ---------------- interior evaluation (FP64) -----------------------------
+ * Point observers against source triangles, for InteriorOp
+ * (tectosaur/interior.py:29-119).
+ *
+ * tct_interior_pairs: interior_pairs (tectosaur/assemble.cl:140-186) via
+ * integrate_pt_tri (tectosaur/integral_primitives.cl:216-254), called by
+ * interior_pairs_quad (tectosaur/interior.py:121-142).
+ *   kernel: U, T, A, H or RT (no observation surface curl);
+ *   qx [nq][2] f64 (src xhat, yhat), qw [nq] f64;
+ *   obs_pts, obs_ns [n_obs][3] f64; src_pts [n][3] f64, src_tris [n][3] int32;
+ *   pairs [n_pairs][3] int32 = (obs point, src triangle, src clicks);
+ *   check0: skip r2 == 0 (the point-farfield form; interior_pairs does not);
+ *   finite_part: write only column b_src = clicks (others zero);
+ *   out [n_pairs][27] f64, index d_obs*9 + b_src*3 + d_src, times J_src.
+ * lanes_per_pair (1, 4, 16 or 64) or 0 to pick from nq. */
+int tct_interior_pairs(int kernel, int64_t nq, const double *qx, const double *qw,
+                       const double *obs_pts, const double *obs_ns, const double *src_pts,
+                       const int32_t *src_tris, int64_t n_pairs, const int32_t *pairs,
+                       int check0, int finite_part, const double *params_host,
+                       int lanes_per_pair, double *out, void *stream);
+
+/* tct_interior_corners: interior_corners (tectosaur/interior_corners.cl
+ * :23-175), called by InteriorOp.build_vertex_mat (tectosaur/interior.py
+ * :66-95): observation point at vertex `clicks` of the source triangle.
+ *   qt, qw [nq]: the angle rule on [0, pi/2]; pairs [n_pairs][3] int32 =
+ *   (obs point, src triangle, vertex); out [n_pairs][27] f64 as above.
+ * Uses elasticH3's constants whatever the caller's kernel, as that file
+ * does. */
+int tct_interior_corners(int64_t nq, const double *qt, const double *qw, const double *obs_ns,
+                         const double *src_pts, const int32_t *src_tris, int64_t n_pairs,
+                         const int32_t *pairs, const double *params_host, double *out,
+                         void *stream);
+
+/* tct_bsr39_spmv: y[3i+d] (+)= sum_k data[k][d][j] x[9 indices[k] + j] over
+ * the 3 x 9 blocks of block-row i -- the pair matrices of make_pairs_mat
+ * (tectosaur/interior.py:14-26) applied in InteriorOp.dot (:108-114).
+ * indptr [n_block_rows+1] int32, indices [n_blocks] int32,
+ * data [n_blocks][27] f64. */
+int tct_bsr39_spmv(int64_t n_block_rows, const int32_t *indptr, const int32_t *indices,
+                   const double *data, const double *x, double *y, int accumulate,
+                   void *stream);
+
 /* ---------------- block sparse matvec (FP64) -----------------------------
  * Reference: bsrmv9 / bcoomv9 (tectosaur/util/fast_sparse.cpp:71-163).
  * y[9r+i] (+)= sum_k A_k[i][j] x[9 col_k + j] over the blocks of block-row r.

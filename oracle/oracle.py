@@ -30,6 +30,19 @@ Pinning (what fixes each function independently of this restatement):
       assertions of tectosaur tests/test_find_near_adj.py:18-37.
   bcoo_matvec                     dense matrix-vector product
       (tectosaur tests/test_sparse.py:8-73 semantics).
+  interior_pairs (U/T/A/H)        golden master tests/golden_masters/
+      test_interior.npy (H, nq=4 over a 2x2 plate, obs normals from the
+      seeded RNG of tectosaur tests/test_integral_op.py:200-218) to 2e-7;
+      equals farfield_pts over the gathered quadrature points (a separate
+      code path); T on a closed sphere reproduces the rigid-body identity
+      (-u inside, 0 outside).
+  interior_pairs (RT)             the integration-by-parts identity: summed
+      over a closed surface, RT equals T for a continuous P1 input.
+  interior_corners                parity unpinned: the reference's corner
+      integrals have no fixture and no closed form fixes their finite-part
+      convention; checked only against the HIP restatement.
+  interior_op_dot                 composed from the pinned pieces above;
+      the test_interior.npy fixture pins it whole when nq_far == nq_near.
 """
 import ctypes
 import os
@@ -70,6 +83,10 @@ def lib():
                                                ctypes.c_double, _i64, ctypes.c_int64]
         L.or_near_pairs_bruteforce.restype = ctypes.c_int64
         L.or_bcoo_matvec.argtypes = [ctypes.c_int64, ctypes.c_int, _i64, _i64, _d, _d, _d]
+        L.or_interior_pairs.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d, _d, _d, _i32,
+                                        ctypes.c_int64, _i64, ctypes.c_int, ctypes.c_int, _d, _d]
+        L.or_interior_corners.argtypes = [ctypes.c_int64, _d, _d, _d, _d, _i32, ctypes.c_int64,
+                                          _i64, _d, _d]
         _LIB = L
     return _LIB
 
@@ -259,3 +276,103 @@ def nearfield_blocks(pts, tris, nq_co, nq_ea, nq_va, nq_far, nq_near, near_thres
                                prs[:, :2], name == 'co', params)
         out[name] = (prs, (acc - corr).reshape((-1, 9, 9)), acc.reshape((-1, 9, 9)))
     return out
+
+
+def interior_pairs(K, qx, qw, obs_pts, obs_ns, pts, tris, pairs, check0, finite_part, params):
+    """Point-observer pair integrals (interior_pairs, tectosaur/assemble.cl
+    :140-186): [n, 3, 3, 3] = [d_obs, b_src, d_src]; pairs [n, 3] =
+    (obs point, src triangle, src clicks)."""
+    qx, qw, obs_pts, obs_ns, pts, params = (_f64(a) for a in (qx, qw, obs_pts, obs_ns, pts,
+                                                               params))
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    pairs = np.ascontiguousarray(np.asarray(pairs).reshape((-1, 3)), dtype=np.int64)
+    n = pairs.shape[0]
+    out = np.zeros(n * 27)
+    if n > 0:
+        lib().or_interior_pairs(KERNEL_IDS[K], qw.shape[0], _p(qx), _p(qw), _p(obs_pts),
+                                _p(obs_ns), _p(pts), _p(tris, _i32), n, _p(pairs, _i64),
+                                int(bool(check0)), int(bool(finite_part)), _p(params), _p(out))
+    return out.reshape((n, 3, 3, 3))
+
+
+def interior_corners(qt, qw, obs_ns, pts, tris, pairs, params):
+    """Observation point at a vertex of the source triangle
+    (tectosaur/interior_corners.cl:23-175): [n, 3, 3, 3]."""
+    qt, qw, obs_ns, pts, params = (_f64(a) for a in (qt, qw, obs_ns, pts, params))
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    pairs = np.ascontiguousarray(np.asarray(pairs).reshape((-1, 3)), dtype=np.int64)
+    n = pairs.shape[0]
+    out = np.zeros(n * 27)
+    if n > 0:
+        lib().or_interior_corners(qw.shape[0], _p(qt), _p(qw), _p(obs_ns), _p(pts),
+                                  _p(tris, _i32), n, _p(pairs, _i64), _p(params), _p(out))
+    return out.reshape((n, 3, 3, 3))
+
+
+def near_pt_pairs(obs_pts, pts, tris, threshold):
+    """(obs point, triangle) pairs with |x - c|^2 <= (r thr)^2, brute force
+    (get_nearfield with zero observation radius, tectosaur/interior.py:33-37)."""
+    cb, rb = tri_centroids_rs(pts, tris)
+    ca = _f64(obs_pts)
+    ra = np.zeros(ca.shape[0])
+    cb, rb = _f64(cb), _f64(rb)
+    cap = 1 << 16
+    while True:
+        out = np.zeros(2 * cap, dtype=np.int64)
+        k = lib().or_near_pairs_bruteforce(ca.shape[0], _p(ca), _p(ra), cb.shape[0], _p(cb),
+                                           _p(rb), float(threshold), _p(out, _i64), cap)
+        if k <= cap:
+            return out[:2 * k].reshape((k, 2))
+        cap = k
+
+
+def split_vertex_nearfield(pairs, obs_pts, pts, tris):
+    """tectosaur/mesh/fast_find_nearfield.cpp:184-215: (vertex pairs [n,3] =
+    (obs, tri, last vertex d within 1e-9 of the point), other pairs [n,3] =
+    (obs, tri, 0)), each in input order."""
+    vert, near = [], []
+    for i, j in pairs:
+        isv = -1
+        for d in range(3):
+            if np.linalg.norm(obs_pts[i] - pts[tris[j, d]]) < 1e-9:
+                isv = d
+        (vert if isv >= 0 else near).append((i, j, max(isv, 0)))
+    return (np.array(vert, dtype=np.int64).reshape((-1, 3)),
+            np.array(near, dtype=np.int64).reshape((-1, 3)))
+
+
+def pairs_matvec(pairs, entries, v, n_obs):
+    """y[3 obs + d] += sum entries[p, d, k] v[9 tri + k] (make_pairs_mat of
+    tectosaur/interior.py:14-26 applied to v)."""
+    y = np.zeros(n_obs * 3)
+    e = np.asarray(entries).reshape((-1, 3, 9))
+    for p, (i, j) in enumerate(np.asarray(pairs)[:, :2]):
+        y[3 * i:3 * i + 3] += e[p] @ v[9 * j:9 * j + 9]
+    return y
+
+
+def interior_op_dot(obs_pts, obs_ns, pts, tris, K, v, params, far_rule, near_rule,
+                    vertex_rule, threshold=4.0):
+    """InteriorOp.dot of tectosaur/interior.py:29-119, dense on the CPU:
+    far (every pair, far rule, r2 == 0 skipped) + near pairs (near rule)
+    - every close pair (far rule) + vertex pairs (corner rule).  The rules
+    are (points, weights) so this module does not import the product."""
+    obs_pts, obs_ns, pts, v = (_f64(a) for a in (obs_pts, obs_ns, pts, v))
+    n_obs, n_tris = obs_pts.shape[0], tris.shape[0]
+    allp = np.stack(np.meshgrid(np.arange(n_obs), np.arange(n_tris), indexing='ij'),
+                    axis=2).reshape((-1, 2))
+    allp = np.concatenate([allp, np.zeros((allp.shape[0], 1), np.int64)], axis=1)
+    far = interior_pairs(K, far_rule[0], far_rule[1], obs_pts, obs_ns, pts, tris, allp, True,
+                         False, params)
+    y = pairs_matvec(allp, far, v, n_obs)
+    close = near_pt_pairs(obs_pts, pts, tris, threshold)
+    vert, near = split_vertex_nearfield(close, obs_pts, pts, tris)
+    every = np.concatenate([near, vert], axis=0)
+    every[:, 2] = 0
+    y += pairs_matvec(near, interior_pairs(K, near_rule[0], near_rule[1], obs_pts, obs_ns, pts,
+                                           tris, near, False, False, params), v, n_obs)
+    y -= pairs_matvec(every, interior_pairs(K, far_rule[0], far_rule[1], obs_pts, obs_ns, pts,
+                                            tris, every, False, False, params), v, n_obs)
+    y += pairs_matvec(vert, interior_corners(vertex_rule[0], vertex_rule[1], obs_ns, pts, tris,
+                                             vert, params), v, n_obs)
+    return y

@@ -657,3 +657,158 @@ void or_bcoo_matvec(int64_t n_blocks, int bs, const int64_t *rows, const int64_t
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Interior evaluation (point observers).                              */
+/* interior_pairs, tectosaur/assemble.cl:140-186, through               */
+/* integrate_pt_tri (tectosaur/integral_primitives.cl:216-254): the      */
+/* tensor code with one observation "basis" obsb = {1}; for RT the        */
+/* regularized tensor elasticRTA3_tensor('obs', 'src', 1)               */
+/* (tectosaur/kernels/kernels.cl:29-62).  pairs[n][3] = (obs point,     */
+/* src triangle, src rotation clicks).  check0 skips r2 == 0, as the    */
+/* point farfield farfield_tris_to_pts does (matrix_free.cl:165-167);   */
+/* interior_pairs itself does not.  finite_part writes only column      */
+/* b_src = clicks (assemble.cl:164-172), the rest stay zero.            */
+/* out[n*27], index d_obs*9 + b_src*3 + d_src, times J_src.              */
+/* ------------------------------------------------------------------ */
+static void pt_integrand(int kid, const double *x, const double *m, const tri_t *st,
+                         const double *q, double w, int check0, const double *params,
+                         acc_t *res)
+{
+    double srcb[3], y[3];
+    basis3(q[0], q[1], srcb);
+    interp(st, srcb, y);
+    double D[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+    double r2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+    if (check0 && r2 == 0.0)
+        return;
+    if (kid <= K_H) {
+        double K[9];
+        kernel_tensor(kid, D, r2, m, st->n, params, K);
+        for (int dob = 0; dob < 3; dob++)
+            for (int bs = 0; bs < 3; bs++)
+                for (int ds = 0; ds < 3; ds++)
+                    res[dob * 9 + bs * 3 + ds] += w * srcb[bs] * K[dob * 3 + ds];
+        return;
+    }
+    /* RT */
+    rconst_t c = rconsts(params);
+    double invr = 1.0 / sqrt(r2);
+    double invr3 = invr * invr * invr;
+    double Q1 = c.CsRT1 * invr;
+    double Q2 = c.CsRT0 * invr3;
+    double Q3nD = c.CsRT2 * invr3 * (st->n[0] * D[0] + st->n[1] * D[1] + st->n[2] * D[2]);
+    for (int dob = 0; dob < 3; dob++)
+        for (int ds = 0; ds < 3; ds++)
+            for (int bs = 0; bs < 3; bs++) {
+                double Kval = 0.0;
+                for (int Ij = 0; Ij < 3; Ij++)
+                    Kval += Gimk(dob, Ij, ds, Q1, Q2, D) * st->curl[bs][Ij];
+                if (dob == ds)
+                    Kval += Q3nD * srcb[bs];
+                res[dob * 9 + bs * 3 + ds] += w * Kval;
+            }
+}
+
+static void pt_derotate(const acc_t *res, double J, int clicks, int finite_part, double *o)
+{
+    int derot[3];
+    for (int i = 0; i < 3; i++)
+        derot[i] = pmod3(-clicks + i);
+    for (int k = 0; k < 27; k++)
+        o[k] = 0.0;
+    for (int d1 = 0; d1 < 3; d1++)
+        for (int b2 = 0; b2 < 3; b2++) {
+            if (finite_part && b2 != clicks)
+                continue;
+            for (int d2 = 0; d2 < 3; d2++)
+                o[d1 * 9 + b2 * 3 + d2] = J * (double)res[d1 * 9 + derot[b2] * 3 + d2];
+        }
+}
+
+void or_interior_pairs(int kid, int64_t n_q, const double *qx, const double *qw,
+                       const double *obs_pts, const double *obs_ns, const double *src_pts,
+                       const int32_t *src_tris, int64_t n_pairs, const int64_t *pairs,
+                       int check0, int finite_part, const double *params, double *out)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < n_pairs; p++) {
+        const int64_t *pr = pairs + 3 * p;
+        int clicks = (int)pr[2];
+        tri_t st;
+        tri_info(&st, src_pts, src_tris, pr[1], clicks, 0);
+        const double *x = obs_pts + 3 * pr[0];
+        const double *m = obs_ns + 3 * pr[0];
+        acc_t res[27];
+        for (int k = 0; k < 27; k++)
+            res[k] = 0;
+        for (int64_t iq = 0; iq < n_q; iq++)
+            pt_integrand(kid, x, m, &st, qx + 2 * iq, qw[iq], check0, params, res);
+        pt_derotate(res, st.jac, clicks, finite_part, out + 27 * p);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* interior_corners, tectosaur/interior_corners.cl:23-175: observation  */
+/* point at vertex `clicks` of the source triangle, a 1-D rule in the   */
+/* angle t over [0, pi/2] (interior.py:66-69) and the radial integrals  */
+/* I1, I2, I3 of that file in closed form.  The kernel is elasticH3's   */
+/* constants whatever operator the caller builds (the .cl file fixes    */
+/* K = elasticH3), and only the terms left active there are summed:     */
+/* CsH0 CsH3 mn T1 + delta CsH0 CsH1 mn T0.  The I4 entries of that     */
+/* file are written as `1/3*(...)` and `-1/3*(...)`; C integer division */
+/* makes both factors zero, so I4 = 0 there and here.                   */
+/* pairs[n][3] = (obs point, src triangle, vertex index); out[n*27].     */
+/* ------------------------------------------------------------------ */
+void or_interior_corners(int64_t n_q, const double *qt, const double *qw,
+                         const double *obs_ns, const double *src_pts,
+                         const int32_t *src_tris, int64_t n_pairs, const int64_t *pairs,
+                         const double *params, double *out)
+{
+    double G = params[0], nu = params[1];
+    double CsH0 = G / (4 * M_PI * (1 - nu));
+    double CsH1 = 1 - 2 * nu;
+    double CsH3 = 3 * nu;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t p = 0; p < n_pairs; p++) {
+        const int64_t *pr = pairs + 3 * p;
+        int clicks = (int)pr[2];
+        tri_t st;
+        tri_info(&st, src_pts, src_tris, pr[1], clicks, 0);
+        const double *m = obs_ns + 3 * pr[0];
+        double v1[3], v2[3];
+        for (int d = 0; d < 3; d++) {
+            v1[d] = st.v[1][d] - st.v[0][d];
+            v2[d] = st.v[2][d] - st.v[0][d];
+        }
+        double mn = m[0] * st.n[0] + m[1] * st.n[1] + m[2] * st.n[2];
+        acc_t res[27];
+        for (int k = 0; k < 27; k++)
+            res[k] = 0;
+        for (int64_t iq = 0; iq < n_q; iq++) {
+            double t = qt[iq], w = qw[iq];
+            double ct = cos(t), s = sin(t);
+            double rhomax = 1.0 / (ct + s);
+            double a[3];
+            for (int d = 0; d < 3; d++)
+                a[d] = v1[d] * ct + v2[d] * s;
+            double A2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+            double A = sqrt(A2);
+            double A3 = A2 * A, A5 = A3 * A2;
+            double I1 = -1.0 / (A3 * rhomax);
+            double I2 = (log(2.0) + log(A) + log(rhomax) - 1.0) / A3;
+            double T0[3] = {-I2 * ct - I2 * s + I1, I2 * ct, I2 * s};
+            for (int dob = 0; dob < 3; dob++)
+                for (int b = 0; b < 3; b++)
+                    for (int ds = 0; ds < 3; ds++) {
+                        /* T1basis[0] = I3, T1basis[1] = T1basis[2] = 0 (I4 = 0) */
+                        double I3 = -a[dob] * a[ds] / (A5 * rhomax);
+                        double K = b == 0 ? CsH0 * CsH3 * I3 * mn : 0.0;
+                        if (dob == ds)
+                            K += CsH0 * CsH1 * mn * T0[b];
+                        res[dob * 9 + b * 3 + ds] += w * K;
+                    }
+        }
+        pt_derotate(res, st.jac, clicks, 0, out + 27 * p);
+    }
+}

@@ -62,6 +62,10 @@ SIGNATURES = {
                             _vp, _vp],
     'tct_sub_f64': [_i64, _vp, _vp, _vp, _vp],
     'tct_bsr9_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
+    'tct_interior_pairs': [_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _int, _int, _vp,
+                           _int, _vp, _vp],
+    'tct_interior_corners': [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    'tct_bsr39_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_bsr9_spmv_f32': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_csr_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_find_near_pairs': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _vp, _i64, _vp],
