@@ -196,6 +196,79 @@ def treecode_bench(pts, tris, v, settings=((3, 2.0), (4, 2.5))):
     return res
 
 
+def interior_bench(n_grid=250, nq_far=3, nq_near=8, nq_vertex=8, reps=5):
+    """Interior evaluation (InteriorOp): displacement (elasticT3) of the
+    2,048-triangle vertical fault (x in [-1000, 1000] m, z in [-2000, 0] m,
+    33 x 33 points) on an n_grid x n_grid horizontal grid at z = -500 m over
+    [-3000, 3000]^2 m that crosses the fault; slip N(0, 1) from
+    default_rng(2).  FP32 farfield, FP64 pair blocks."""
+    import torch
+    from tectosaur_amd import mesh
+    from tectosaur_amd.interior import InteriorOp
+    pts, tris = mesh.make_rect(33, 33, [[-1000, 0, 0], [1000, 0, 0], [1000, 0, -2000],
+                                        [-1000, 0, -2000]])
+    xs = np.linspace(-3000, 3000, n_grid)
+    X, Y = np.meshgrid(xs, xs)
+    obs = np.stack([X.ravel(), Y.ravel(), np.full(X.size, -500.0)], axis=1)
+    ns = np.tile([0.0, 0.0, 1.0], (obs.shape[0], 1))
+    v = np.random.default_rng(2).standard_normal(tris.shape[0] * 9)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    op = InteriorOp(obs, ns, (pts, tris), 'elasticT3', 4.0, nq_vertex, nq_far, nq_near, PARAMS,
+                    np.float32)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    v_dev = torch.as_tensor(v, dtype=torch.float64, device='cuda')
+    out = torch.empty(op.shape[0], dtype=torch.float64, device='cuda')
+    op.dot_dev(v_dev, out)
+    ms = 1e3 * stream_event_time(lambda: op.dot_dev(v_dev, out), reps)
+    far_ms = 1e3 * stream_event_time(lambda: op.farfield.dot_dev(v_dev, out), reps)
+    pe = op.pair_evals
+    res = {'workload': 'elasticT3 of a 2,048-tri vertical fault at a %dx%d grid (%d points) '
+                       'crossing it; far rule %d (%d pts), near rule %d, vertex rule %d, '
+                       'threshold 4' % (n_grid, n_grid, obs.shape[0], nq_far,
+                                        op.farfield.nq, nq_near, nq_vertex),
+           'close_pairs': int(op.n_blocks), 'vertex_pairs': int(op.vertex_pairs.shape[0]),
+           'setup_s': t_build, 'dot_ms': ms, 'farfield_ms': far_ms,
+           'farfield_pair_evals_per_s': pe / (far_ms * 1e-3),
+           'points_per_s': obs.shape[0] / (ms * 1e-3)}
+    print('# interior: %s' % json.dumps(res), flush=True)
+    return res
+
+
+def pairfind_bench(pts, tris, thr=2.5, reps=3):
+    """Near-pair finding and adjacency classification on the two-surface
+    mesh (147,456 triangles, threshold 2.5): the device finder against the
+    host C++/OpenMP finder; both must return the same pairs."""
+    import torch
+    from tectosaur_amd import find_near_adj as F
+    ca, ra = F.get_tri_centroids_rs(pts, tris)
+    t_dev = torch.as_tensor(np.ascontiguousarray(tris, dtype=np.int64), device='cuda')
+    F.find_near_pairs_dev(ca, ra, ca, ra, thr)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        d = F.find_near_pairs_dev(ca, ra, ca, ra, thr)
+        cls_d = F.split_adjacent_close_dev(d, t_dev, t_dev)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    th = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        h = F.find_near_pairs(ca, ra, ca, ra, thr)
+        cls_h = F.split_adjacent_close(h, tris, tris)
+        th.append(time.perf_counter() - t0)
+    same = bool(np.array_equal(d.cpu().numpy(), h) and
+                all(np.array_equal(a.cpu().numpy(), b) for a, b in zip(cls_d, cls_h)))
+    res = {'workload': 'two-surface mesh, %d tris, threshold %g' % (tris.shape[0], thr),
+           'pairs': int(h.shape[0]), 'classes': [int(x.shape[0]) for x in cls_h],
+           'device_s': min(ts), 'host_s': min(th), 'host_threads': os.cpu_count(),
+           'identical': same}
+    print('# pair finding: %s' % json.dumps(res), flush=True)
+    return res
+
+
 def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
     """The CPU oracle (C restatement of farfield_tris_to_tris, FP64 with
     long-double accumulation, OpenMP) on a bounded sample of observation
@@ -270,6 +343,8 @@ def main():
     ap.add_argument('--no-serve', action='store_true')
     ap.add_argument('--no-nearfield', action='store_true')
     ap.add_argument('--no-treecode', action='store_true')
+    ap.add_argument('--no-interior', action='store_true')
+    ap.add_argument('--no-pairfind', action='store_true')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run a 1 GiB device copy first (scale for byte counters)')
     args = ap.parse_args()
@@ -356,6 +431,10 @@ def main():
         line['treecode'] = treecode_bench(pts, tris, v)
     if not args.no_nearfield:
         line['nearfield'] = nearfield_bench()
+    if not args.no_interior:
+        line['interior'] = interior_bench()
+    if not args.no_pairfind:
+        line['pair_finding'] = pairfind_bench(pts, tris)
     if not args.no_serve:
         line['served'] = serve_bench(args.serve_requests, args.serve_max_tris, args.serve_rate)
     if not args.no_cpu_baseline:

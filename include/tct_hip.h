@@ -305,6 +305,43 @@ int tct_find_near_pairs(int64_t n_a, const double *ca, const double *ra, int64_t
                         const double *cb, const double *rb, double threshold, int64_t *out,
                         int64_t cap, int64_t *n_out);
 
+/* ---------------- device near-pair finder and classifier ----------------
+ * Reference: fast_find_nearfield.get_nearfield and split_adjacent_close
+ * (tectosaur/mesh/fast_find_nearfield.cpp:22-182), called through
+ * find_close_or_touching (tectosaur/mesh/find_near_adj.py:29-35).  Same pair
+ * set and order as tct_find_near_pairs (bit-identical predicate).
+ *
+ * tct_near_grid_params (host): grid over the B centroids cb [n_b][3] (host)
+ *   for a largest reach reach_max = thr (rmax_a + rmax_b) 1.0000001:
+ *   lo[3], cell, dims[3].
+ * tct_near_pairs_dev_scratch (host): scratch bytes for n_a, n_b and dims.
+ * tct_near_pairs_dev_count: ca, ra [n_a][3]/[n_a], cb, rb (device f64);
+ *   builds the grid in scratch and writes offsets [n_a + 1] int64 (device;
+ *   offsets[n_a] = number of pairs).
+ * tct_near_pairs_dev_fill: same arguments and scratch, after _count on the
+ *   same stream: out [offsets[n_a]][2] int64 (device), sorted by (i, j).
+ * tct_split_adjacent_close_dev: pairs [n][2] int64, tris_a/tris_b [.][3]
+ *   int64 (device) -> near [n][2], va [n][4], ea [n][6] (capacity n) and
+ *   counts [3] int64 (device), in input order; coincident pairs dropped.
+ *   Scratch size from tct_split_adjacent_dev_scratch. */
+int tct_near_grid_params(int64_t n_b, const double *cb, double reach_max, double *lo,
+                         double *cell, int64_t *dims);
+int tct_near_pairs_dev_scratch(int64_t n_a, int64_t n_b, const int64_t *dims, size_t *bytes);
+int tct_near_pairs_dev_count(int64_t n_a, const double *ca, const double *ra, int64_t n_b,
+                             const double *cb, const double *rb, double thr, double rmax_b,
+                             const double *lo, double cell, const int64_t *dims, void *scratch,
+                             size_t scratch_bytes, int64_t *offsets, void *stream);
+int tct_near_pairs_dev_fill(int64_t n_a, const double *ca, const double *ra, int64_t n_b,
+                            const double *cb, const double *rb, double thr, double rmax_b,
+                            const double *lo, double cell, const int64_t *dims, void *scratch,
+                            size_t scratch_bytes, const int64_t *offsets, int64_t *out,
+                            void *stream);
+int tct_split_adjacent_dev_scratch(int64_t n_pairs, size_t *bytes);
+int tct_split_adjacent_close_dev(int64_t n_pairs, const int64_t *pairs, const int64_t *tris_a,
+                                 const int64_t *tris_b, int64_t *near, int64_t *va, int64_t *ea,
+                                 int64_t *counts, void *scratch, size_t scratch_bytes,
+                                 void *stream);
+
 /* tct_split_adjacent_close: classify near pairs by shared vertices
  * (tectosaur/mesh/fast_find_nearfield.cpp:178-217).  pairs [n][2] index
  * tris_a / tris_b ([*][3] int64).  Writes, in input order, near [*][2]

@@ -66,6 +66,15 @@ SIGNATURES = {
                            _int, _vp, _vp],
     'tct_interior_corners': [_i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     'tct_bsr39_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
+    'tct_near_grid_params': [_i64, _vp, _dbl, _vp, _vp, _vp],
+    'tct_near_pairs_dev_scratch': [_i64, _i64, _vp, _vp],
+    'tct_near_pairs_dev_count': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp, _vp,
+                                 ctypes.c_size_t, _vp, _vp],
+    'tct_near_pairs_dev_fill': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp, _vp,
+                                ctypes.c_size_t, _vp, _vp, _vp],
+    'tct_split_adjacent_dev_scratch': [_i64, _vp],
+    'tct_split_adjacent_close_dev': [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                     ctypes.c_size_t, _vp],
     'tct_bsr9_spmv_f32': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_csr_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_find_near_pairs': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _vp, _i64, _vp],

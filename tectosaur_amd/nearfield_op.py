@@ -68,10 +68,12 @@ def to_dof_space(tri_indices, obs_subset, src_subset):
     return out
 
 
-def nearfield_plan(pts, tris, obs_subset, src_subset, near_threshold):
-    """Host-side pair classification (no device work): coincident,
-    edge-adjacent (rotated), vertex-adjacent and near pairs, in triangle and
-    DOF space (nearfield_op.py:95-114)."""
+def nearfield_plan(pts, tris, obs_subset, src_subset, near_threshold, finder='host'):
+    """Pair classification: coincident, edge-adjacent (rotated),
+    vertex-adjacent and near pairs, in triangle and DOF space
+    (nearfield_op.py:95-114).  finder 'host' searches with the C++ finder
+    (no device work, safe on worker threads); 'device' with the device
+    finder and classifier (the same pairs, in the same order)."""
     pts = np.asarray(pts, dtype=np.float64)
     tris = np.asarray(tris, dtype=np.int64)
     obs_subset = np.asarray(obs_subset, dtype=np.int64)
@@ -79,10 +81,22 @@ def nearfield_plan(pts, tris, obs_subset, src_subset, near_threshold):
     co_tris = np.intersect1d(obs_subset, src_subset)
     co_indices = np.stack([co_tris, co_tris], axis=1)
     co_dofs = to_dof_space(co_indices, obs_subset, src_subset)
-    close = find_near_adj.find_close_or_touching(pts, tris[obs_subset], pts, tris[src_subset],
-                                                 near_threshold)
-    near_dofs, va_dofs, ea_dofs = find_near_adj.split_adjacent_close(
-        close, tris[obs_subset], tris[src_subset])
+    if finder == 'device':
+        import torch
+        ta, tb = tris[obs_subset], tris[src_subset]
+        close = find_near_adj.find_near_pairs_dev(
+            *find_near_adj.get_tri_centroids_rs(pts, ta),
+            *find_near_adj.get_tri_centroids_rs(pts, tb), near_threshold)
+        d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=close.device)
+        near_dofs, va_dofs, ea_dofs = (x.cpu().numpy() for x in
+                                       find_near_adj.split_adjacent_close_dev(close, d(ta), d(tb)))
+    elif finder == 'host':
+        close = find_near_adj.find_close_or_touching(pts, tris[obs_subset], pts,
+                                                     tris[src_subset], near_threshold)
+        near_dofs, va_dofs, ea_dofs = find_near_adj.split_adjacent_close(
+            close, tris[obs_subset], tris[src_subset])
+    else:
+        raise ValueError("finder must be 'host' or 'device'")
     near = to_tri_space(near_dofs, obs_subset, src_subset)
     va = to_tri_space(va_dofs, obs_subset, src_subset)
     va = np.concatenate([va, np.zeros((va.shape[0], 1), dtype=np.int64)], axis=1)
