@@ -249,6 +249,19 @@ int tct_interior_pairs(int kernel, int64_t nq, const double *qx, const double *q
                        int check0, int finite_part, const double *params_host,
                        int lanes_per_pair, double *out, void *stream);
 
+/* tct_interior_pairs_sinh: as tct_interior_pairs for near-singular pairs,
+ * with a polar sinh rule about the point of the triangle closest to the
+ * projection of the observation point (no reference counterpart: the
+ * reference uses a fixed Gauss rule, which loses accuracy as the point
+ * nears the triangle).  tq, tw [n_t] and rq, rw [n_r]: Gauss rules on
+ * [0, 1] for the angle and the sinh variable (6 n_t n_r points per pair).
+ * pairs and out as tct_interior_pairs (physical weights, J included). */
+int tct_interior_pairs_sinh(int kernel, int n_t, const double *tq, const double *tw, int n_r,
+                            const double *rq, const double *rw, const double *obs_pts,
+                            const double *obs_ns, const double *src_pts,
+                            const int32_t *src_tris, int64_t n_pairs, const int32_t *pairs,
+                            const double *params_host, double *out, void *stream);
+
 /* tct_interior_corners: interior_corners (tectosaur/interior_corners.cl
  * :23-175), called by InteriorOp.build_vertex_mat (tectosaur/interior.py
  * :66-95): observation point at vertex `clicks` of the source triangle.
@@ -269,6 +282,16 @@ int tct_interior_corners(int64_t nq, const double *qt, const double *qw, const d
 int tct_bsr39_spmv(int64_t n_block_rows, const int32_t *indptr, const int32_t *indices,
                    const double *data, const double *x, double *y, int accumulate,
                    void *stream);
+
+/* tct_limit_combine: out[i][c] = sum_k w[i][k] vals[k][i][c] -- the
+ * extrapolation to the surface of a field sampled at n_eps offsets along
+ * each point's normal (w: per-point least-squares fit weights of the
+ * polynomial + log model of tectosaur tests/richardson_error.py:4-21).
+ * w [n_pts][n_eps], vals [n_eps][n_pts][comps], out [n_pts][comps], f64.
+ * No reference counterpart in the library (the fit exists there only as
+ * that test script). */
+int tct_limit_combine(int n_eps, int64_t n_pts, int comps, const double *w, const double *vals,
+                      double *out, void *stream);
 
 /* ---------------- block sparse matvec (FP64) -----------------------------
  * Reference: bsrmv9 / bcoomv9 (tectosaur/util/fast_sparse.cpp:71-163).

@@ -38,6 +38,10 @@ Pinning (what fixes each function independently of this restatement):
       (-u inside, 0 outside).
   interior_pairs (RT)             the integration-by-parts identity: summed
       over a closed surface, RT equals T for a continuous P1 input.
+  interior_pairs_sinh             equals the converged sum over a 4^6-way
+      subdivision of the triangle (brute force) for points above, beside
+      and near the edges of a triangle; the rigid-body identity at points
+      1e-3 triangle sizes from a closed sphere.
   interior_corners                parity unpinned: the reference's corner
       integrals have no fixture and no closed form fixes their finite-part
       convention; checked only against the HIP restatement.
@@ -85,6 +89,9 @@ def lib():
         L.or_bcoo_matvec.argtypes = [ctypes.c_int64, ctypes.c_int, _i64, _i64, _d, _d, _d]
         L.or_interior_pairs.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, _d, _d, _d, _i32,
                                         ctypes.c_int64, _i64, ctypes.c_int, ctypes.c_int, _d, _d]
+        L.or_interior_pairs_sinh.argtypes = [ctypes.c_int, ctypes.c_int64, _d, _d, ctypes.c_int64,
+                                             _d, _d, _d, _d, _d, _i32, ctypes.c_int64, _i64, _d,
+                                             _d]
         L.or_interior_corners.argtypes = [ctypes.c_int64, _d, _d, _d, _d, _i32, ctypes.c_int64,
                                           _i64, _d, _d]
         _LIB = L
@@ -295,6 +302,24 @@ def interior_pairs(K, qx, qw, obs_pts, obs_ns, pts, tris, pairs, check0, finite_
     return out.reshape((n, 3, 3, 3))
 
 
+def interior_pairs_sinh(K, t_rule, r_rule, obs_pts, obs_ns, pts, tris, pairs, params):
+    """Point-triangle integrals by the polar sinh rule (t_rule, r_rule: Gauss
+    rules on [0, 1] for the angle and the sinh variable): [n, 3, 3, 3],
+    physical weights."""
+    tq, tw = (_f64(a) for a in t_rule)
+    rq, rw = (_f64(a) for a in r_rule)
+    obs_pts, obs_ns, pts, params = (_f64(a) for a in (obs_pts, obs_ns, pts, params))
+    tris = np.ascontiguousarray(tris, dtype=np.int32)
+    pairs = np.ascontiguousarray(np.asarray(pairs).reshape((-1, 3)), dtype=np.int64)
+    n = pairs.shape[0]
+    out = np.zeros(n * 27)
+    if n > 0:
+        lib().or_interior_pairs_sinh(KERNEL_IDS[K], tq.shape[0], _p(tq), _p(tw), rq.shape[0],
+                                     _p(rq), _p(rw), _p(obs_pts), _p(obs_ns), _p(pts),
+                                     _p(tris, _i32), n, _p(pairs, _i64), _p(params), _p(out))
+    return out.reshape((n, 3, 3, 3))
+
+
 def interior_corners(qt, qw, obs_ns, pts, tris, pairs, params):
     """Observation point at a vertex of the source triangle
     (tectosaur/interior_corners.cl:23-175): [n, 3, 3, 3]."""
@@ -352,11 +377,13 @@ def pairs_matvec(pairs, entries, v, n_obs):
 
 
 def interior_op_dot(obs_pts, obs_ns, pts, tris, K, v, params, far_rule, near_rule,
-                    vertex_rule, threshold=4.0):
+                    vertex_rule, threshold=4.0, sinh_rules=None):
     """InteriorOp.dot of tectosaur/interior.py:29-119, dense on the CPU:
     far (every pair, far rule, r2 == 0 skipped) + near pairs (near rule)
     - every close pair (far rule) + vertex pairs (corner rule).  The rules
-    are (points, weights) so this module does not import the product."""
+    are (points, weights) so this module does not import the product;
+    sinh_rules = (angle rule, sinh rule) on [0, 1] integrates the near pairs
+    with the polar sinh rule instead."""
     obs_pts, obs_ns, pts, v = (_f64(a) for a in (obs_pts, obs_ns, pts, v))
     n_obs, n_tris = obs_pts.shape[0], tris.shape[0]
     allp = np.stack(np.meshgrid(np.arange(n_obs), np.arange(n_tris), indexing='ij'),
@@ -369,8 +396,13 @@ def interior_op_dot(obs_pts, obs_ns, pts, tris, K, v, params, far_rule, near_rul
     vert, near = split_vertex_nearfield(close, obs_pts, pts, tris)
     every = np.concatenate([near, vert], axis=0)
     every[:, 2] = 0
-    y += pairs_matvec(near, interior_pairs(K, near_rule[0], near_rule[1], obs_pts, obs_ns, pts,
-                                           tris, near, False, False, params), v, n_obs)
+    if sinh_rules is None:
+        near_e = interior_pairs(K, near_rule[0], near_rule[1], obs_pts, obs_ns, pts, tris, near,
+                                False, False, params)
+    else:
+        near_e = interior_pairs_sinh(K, sinh_rules[0], sinh_rules[1], obs_pts, obs_ns, pts, tris,
+                                     near, params)
+    y += pairs_matvec(near, near_e, v, n_obs)
     y -= pairs_matvec(every, interior_pairs(K, far_rule[0], far_rule[1], obs_pts, obs_ns, pts,
                                             tris, every, False, False, params), v, n_obs)
     y += pairs_matvec(vert, interior_corners(vertex_rule[0], vertex_rule[1], obs_ns, pts, tris,

@@ -812,3 +812,159 @@ void or_interior_corners(int64_t n_q, const double *qt, const double *qw,
         pt_derotate(res, st.jac, clicks, 0, out + 27 * p);
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Near-singular point-triangle integrals by a polar sinh rule.         */
+/* No reference counterpart: the reference integrates close pairs with  */
+/* a fixed triangle Gauss rule.  The source triangle is split at a      */
+/* point p (below) into three sub-                                      */
+/* triangles (p, v_k, v_k+1), and each of those at the foot of the     */
+/* perpendicular from p into two right triangles (signed when the foot  */
+/* falls outside the edge).  A right triangle with legs h (p to foot)   */
+/* and L is integrated in polar coordinates about p: the angle through  */
+/* tan(phi) = sinh(s), s in [0, asinh(L/h)], so rho_max = h cosh(s) and */
+/* the thin-triangle end stays smooth; the radius through rho =         */
+/* d sinh(u) with d = |x - p|, the sinh transformation of Johnston and  */
+/* Elliott (Int. J. Numer. Meth. Engng 62 (2005) 564-578), which        */
+/* removes the near-singular peak of width d.  p is the point of the    */
+/* triangle closest to the projection of x.  tq/tw, rq/rw: Gauss rules  */
+/* on [0, 1] for s and for u.  Weights are physical (no J factor).      */
+/* pairs and out as or_interior_pairs.                                  */
+/* ------------------------------------------------------------------ */
+static void cross3(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+static double dot3(const double *a, const double *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+void or_interior_pairs_sinh(int kid, int64_t n_t, const double *tq, const double *tw,
+                            int64_t n_r, const double *rq, const double *rw,
+                            const double *obs_pts, const double *obs_ns, const double *src_pts,
+                            const int32_t *src_tris, int64_t n_pairs, const int64_t *pairs,
+                            const double *params, double *out)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t p_i = 0; p_i < n_pairs; p_i++) {
+        const int64_t *pr = pairs + 3 * p_i;
+        int clicks = (int)pr[2];
+        tri_t st;
+        tri_info(&st, src_pts, src_tris, pr[1], clicks, 0);
+        const double *x = obs_pts + 3 * pr[0];
+        const double *m = obs_ns + 3 * pr[0];
+        const double *nn = st.n;
+        double xv0[3] = {x[0] - st.v[0][0], x[1] - st.v[0][1], x[2] - st.v[0][2]};
+        double ds = dot3(xv0, nn);
+        double pp[3] = {x[0] - ds * nn[0], x[1] - ds * nn[1], x[2] - ds * nn[2]};
+        /* barycentric map: y - v0 = xi e1 + eta e2 */
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; k++) {
+            e1[k] = st.v[1][k] - st.v[0][k];
+            e2[k] = st.v[2][k] - st.v[0][k];
+        }
+        double g11 = dot3(e1, e1), g12 = dot3(e1, e2), g22 = dot3(e2, e2);
+        double det = g11 * g22 - g12 * g12;
+        /* split point p: the point of the triangle closest to the projection */
+        double p[3] = {pp[0], pp[1], pp[2]};
+        {
+            double pv[3] = {pp[0] - st.v[0][0], pp[1] - st.v[0][1], pp[2] - st.v[0][2]};
+            double h1 = dot3(pv, e1), h2 = dot3(pv, e2);
+            double xi = (g22 * h1 - g12 * h2) / det, eta = (g11 * h2 - g12 * h1) / det;
+            if (xi >= 0 && eta >= 0 && xi + eta <= 1) {
+                for (int k = 0; k < 3; k++)
+                    p[k] = pp[k];
+            } else {
+                double best = INFINITY;
+                for (int ed = 0; ed < 3; ed++) {
+                    const double *va = st.v[ed], *vb = st.v[(ed + 1) % 3];
+                    double ev[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
+                    double w[3] = {pp[0] - va[0], pp[1] - va[1], pp[2] - va[2]};
+                    double t = dot3(w, ev) / dot3(ev, ev);
+                    t = t < 0 ? 0 : (t > 1 ? 1 : t);
+                    double q[3] = {va[0] + t * ev[0], va[1] + t * ev[1], va[2] + t * ev[2]};
+                    double dq[3] = {pp[0] - q[0], pp[1] - q[1], pp[2] - q[2]};
+                    double dd = dot3(dq, dq);
+                    if (dd < best) {
+                        best = dd;
+                        for (int k = 0; k < 3; k++)
+                            p[k] = q[k];
+                    }
+                }
+            }
+        }
+        double xp[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+        double d = sqrt(dot3(xp, xp));
+        double dmin = 1e-12 * sqrt(st.jac);
+        if (d < dmin)
+            d = dmin;
+        acc_t res[27];
+        for (int k = 0; k < 27; k++)
+            res[k] = 0;
+        for (int ed = 0; ed < 3; ed++) {
+            const double *va = st.v[ed], *vb = st.v[(ed + 1) % 3];
+            double a[3], b[3], c[3];
+            for (int k = 0; k < 3; k++) {
+                a[k] = va[k] - p[k];
+                b[k] = vb[k] - p[k];
+            }
+            cross3(a, b, c);
+            double cr = dot3(c, nn);
+            if (fabs(cr) <= 1e-14 * st.jac)
+                continue;
+            double sgn = cr > 0 ? 1.0 : -1.0;
+            /* foot of the perpendicular from p on the edge line */
+            double ev[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
+            double le = sqrt(dot3(ev, ev));
+            double t = -dot3(a, ev) / (le * le);
+            double f[3] = {a[0] + t * ev[0], a[1] + t * ev[1], a[2] + t * ev[2]};
+            double hp = sqrt(dot3(f, f));
+            double ef[3] = {f[0] / hp, f[1] / hp, f[2] / hp};
+            /* right triangles (p, foot, v) for v = va (t_v = 0), vb (t_v = 1) */
+            for (int side = 0; side < 2; side++) {
+                double tv = side ? 1.0 : 0.0;
+                double Lv = (tv - t) * le;
+                if (Lv == 0.0)
+                    continue;
+                double sig = (side ? (1.0 - t) : t) > 0 ? 1.0 : -1.0;
+                double L = fabs(Lv);
+                double et[3];
+                for (int k = 0; k < 3; k++)
+                    et[k] = (Lv > 0 ? 1.0 : -1.0) * ev[k] / le;
+                /* phi from ef: tan(phi) = sinh(s), rho_max = hp cosh(s) */
+                double S = asinh(L / hp);
+                for (int64_t it = 0; it < n_t; it++) {
+                    double sv = S * tq[it];
+                    double chs = cosh(sv), shs = sinh(sv);
+                    double cphi = 1.0 / chs, sphi = shs / chs;
+                    double rmax = hp * chs;
+                    double dir[3];
+                    for (int k = 0; k < 3; k++)
+                        dir[k] = cphi * ef[k] + sphi * et[k];
+                    double U = asinh(rmax / d);
+                    double wphi = S * tw[it] / chs;
+                    for (int64_t ir = 0; ir < n_r; ir++) {
+                        double u = U * rq[ir];
+                        double rho = d * sinh(u);
+                        double w = sgn * sig * wphi * U * rw[ir] * d * cosh(u) * rho;
+                        double y[3], yv[3];
+                        for (int k = 0; k < 3; k++) {
+                            y[k] = p[k] + rho * dir[k];
+                            yv[k] = y[k] - st.v[0][k];
+                        }
+                        double h1 = dot3(yv, e1), h2 = dot3(yv, e2);
+                        double xi = (g22 * h1 - g12 * h2) / det;
+                        double eta = (g11 * h2 - g12 * h1) / det;
+                        double q[2] = {xi, eta};
+                        pt_integrand(kid, x, m, &st, q, w, 0, params, res);
+                    }
+                }
+            }
+        }
+        pt_derotate(res, 1.0, clicks, 0, out + 27 * p_i);
+    }
+}

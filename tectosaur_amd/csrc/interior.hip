@@ -229,6 +229,189 @@ __global__ __launch_bounds__(256) void bsr39_kernel(int64_t n_rows,
     y[r] = accumulate ? y[r] + s : s;
 }
 
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Near-singular pairs by the polar sinh rule (no reference counterpart):
+// the source triangle is split at p, the point of the triangle closest to
+// the projection of x, into sub-triangles (p, v_k, v_k+1), and each of
+// those at the foot of the perpendicular from p into two right triangles
+// (signed when the foot falls outside the edge).  A right triangle with
+// legs h and L is integrated in polar coordinates about p: the angle
+// through tan(phi) = sinh(s), s in [0, asinh(L/h)] (rho_max = h cosh(s),
+// smooth to the thin end), the radius through rho = d sinh(u), d = |x - p|
+// (the sinh transformation of Johnston and Elliott, Int. J. Numer. Meth.
+// Engng 62 (2005) 564-578).  tq/tw and rq/rw are Gauss rules on [0, 1] for
+// s and u.  One wavefront per pair: each right triangle's n_t n_r points
+// are dealt to the lanes; weights are physical (no J factor).
+template <int KID>
+__global__ __launch_bounds__(256) void interior_pairs_sinh_kernel(
+    int n_t, const double *__restrict__ tq, const double *__restrict__ tw, int n_r,
+    const double *__restrict__ rq, const double *__restrict__ rw,
+    const double *__restrict__ obs_pts, const double *__restrict__ obs_ns,
+    const double *__restrict__ src_pts, const int32_t *__restrict__ src_tris, int n_pairs,
+    const int32_t *__restrict__ pairs, ElasticConsts c, double *__restrict__ out) {
+    constexpr int NA = n_acc_pt<KID>();
+    const int pair = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const bool live = pair < n_pairs;
+    const int pr = live ? pair : n_pairs - 1;
+    const int32_t *pp = pairs + (size_t)pr * 3;
+    const int clicks = pp[2];
+    Tri st;
+    load_tri(st, src_pts, src_tris, pp[1], clicks, 0);
+    double x[3], m[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        x[d] = obs_pts[3 * (size_t)pp[0] + d];
+        m[d] = obs_ns[3 * (size_t)pp[0] + d];
+    }
+    const double *nn = st.n;
+    double e1[3], e2[3], pv[3];
+    const double ds = (x[0] - st.v[0][0]) * nn[0] + (x[1] - st.v[0][1]) * nn[1] +
+                      (x[2] - st.v[0][2]) * nn[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e1[k] = st.v[1][k] - st.v[0][k];
+        e2[k] = st.v[2][k] - st.v[0][k];
+        pv[k] = x[k] - ds * nn[k];
+    }
+    const double g11 = dot3(e1, e1), g12 = dot3(e1, e2), g22 = dot3(e2, e2);
+    const double det = g11 * g22 - g12 * g12;
+    // split point: the projection if inside, else the closest boundary point
+    double p[3] = {pv[0], pv[1], pv[2]};
+    {
+        double w[3] = {pv[0] - st.v[0][0], pv[1] - st.v[0][1], pv[2] - st.v[0][2]};
+        const double h1 = dot3(w, e1), h2 = dot3(w, e2);
+        const double xi = (g22 * h1 - g12 * h2) / det, eta = (g11 * h2 - g12 * h1) / det;
+        if (!(xi >= 0 && eta >= 0 && xi + eta <= 1)) {
+            double best = INFINITY;
+            for (int ed = 0; ed < 3; ++ed) {
+                const double *va = st.v[ed], *vb = st.v[(ed + 1) % 3];
+                double ev[3], wa[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    ev[k] = vb[k] - va[k];
+                    wa[k] = pv[k] - va[k];
+                }
+                double t = dot3(wa, ev) / dot3(ev, ev);
+                t = t < 0 ? 0 : (t > 1 ? 1 : t);
+                double dq[3], q[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    q[k] = va[k] + t * ev[k];
+                    dq[k] = pv[k] - q[k];
+                }
+                const double dd = dot3(dq, dq);
+                if (dd < best) {
+                    best = dd;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) p[k] = q[k];
+                }
+            }
+        }
+    }
+    double d;
+    {
+        double xp[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+        d = sqrt(dot3(xp, xp));
+        const double dmin = 1e-12 * sqrt(st.jac);
+        d = d < dmin ? dmin : d;
+    }
+    double acc[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    const int per_rt = n_t * n_r;
+    if (live) {
+#pragma unroll
+        for (int ed = 0; ed < 3; ++ed) {
+            const double *va = st.v[ed], *vb = st.v[(ed + 1) % 3];
+            double a[3], b[3], cr3[3], ev[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a[k] = va[k] - p[k];
+                b[k] = vb[k] - p[k];
+                ev[k] = vb[k] - va[k];
+            }
+            cross3(a, b, cr3);
+            const double cr = dot3(cr3, nn);
+            if (fabs(cr) <= 1e-14 * st.jac) continue;
+            const double sgn = cr > 0 ? 1.0 : -1.0;
+            // foot of the perpendicular from p on the edge line
+            const double le = sqrt(dot3(ev, ev));
+            const double t = -dot3(a, ev) / (le * le);
+            double ef[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ef[k] = a[k] + t * ev[k];
+            const double hp = sqrt(dot3(ef, ef));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ef[k] /= hp;
+            // right triangles (p, foot, v) for v = va (t_v = 0) and vb (t_v = 1)
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const double Lv = ((double)side - t) * le;
+                if (Lv == 0.0) continue;
+                const double sig = (side ? (1.0 - t) : t) > 0 ? 1.0 : -1.0;
+                const double es = Lv > 0 ? 1.0 / le : -1.0 / le;
+                double et[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) et[k] = es * ev[k];
+                const double S = asinh(fabs(Lv) / hp);
+                for (int item = lane; item < per_rt; item += 64) {
+                    const int it = item / n_r, ir = item % n_r;
+                    // tan(phi) = sinh(s): rho_max = hp cosh(s), dphi = ds / cosh(s)
+                    const double sv = S * tq[it];
+                    const double chs = cosh(sv), shs = sinh(sv);
+                    const double cphi = 1.0 / chs, sphi = shs / chs;
+                    const double U = asinh(hp * chs / d);
+                    const double u = U * rq[ir];
+                    const double rho = d * sinh(u);
+                    const double w = sgn * sig * (S * tw[it] / chs) * U * rw[ir] * d * cosh(u) * rho;
+                    double yv[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        yv[k] = p[k] + rho * (cphi * ef[k] + sphi * et[k]) - st.v[0][k];
+                    const double h1 = dot3(yv, e1), h2 = dot3(yv, e2);
+                    const double xi = (g22 * h1 - g12 * h2) / det;
+                    const double eta = (g11 * h2 - g12 * h1) / det;
+                    accumulate_pt<KID>(x, m, st, xi, eta, w, 0, c, acc);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NA; ++k) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[k] = v;
+    }
+    if (!live || lane != 0) return;
+    write_derotated(out + (size_t)pair * 27, clicks, 0, 1.0,
+                    [&](int d1, int b, int d2) { return final_pt<KID>(acc, st, d1, b, d2); });
+}
+
+// out[i][c] = sum_k w[i][k] vals[k][i][c]: the limit (or log coefficient)
+// of a field sampled at n_eps offsets, with per-point fit weights.
+__global__ __launch_bounds__(256) void limit_combine_kernel(int n_eps, int64_t n_pts, int comps,
+                                                            const double *__restrict__ w,
+                                                            const double *__restrict__ vals,
+                                                            double *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_pts * comps) return;
+    const int64_t i = r / comps;
+    double s = 0.0;
+    for (int k = 0; k < n_eps; ++k) s += w[i * n_eps + k] * vals[(size_t)k * n_pts * comps + r];
+    out[r] = s;
+}
+
 template <int KID, int GL>
 static void launch_ip(hipStream_t st, int nq, const double *qx, const double *qw,
                       const double *obs_pts, const double *obs_ns, const double *src_pts,
@@ -328,5 +511,51 @@ extern "C" int tct_bsr39_spmv(int64_t n_block_rows, const int32_t *indptr, const
                        0, as_stream(stream), n_block_rows, indptr, indices, data, x, y,
                        accumulate);
     TCT_CHECK_LAUNCH("bsr39_spmv");
+    return TCT_OK;
+}
+
+extern "C" int tct_limit_combine(int n_eps, int64_t n_pts, int comps, const double *w,
+                                 const double *vals, double *out, void *stream) {
+    TCT_CHECK_ARG(n_eps > 0 && n_eps <= 64 && n_pts >= 0 && comps > 0 && comps <= 81,
+                  "limit_combine: n_eps %d n_pts %lld comps %d", n_eps, (long long)n_pts, comps);
+    TCT_CHECK_ARG(n_pts == 0 || (w && vals && out), "limit_combine: null pointer");
+    if (n_pts == 0) return TCT_OK;
+    hipLaunchKernelGGL(limit_combine_kernel, dim3((unsigned)ceil_div(n_pts * comps, 256)),
+                       dim3(256), 0, as_stream(stream), n_eps, n_pts, comps, w, vals, out);
+    TCT_CHECK_LAUNCH("limit_combine");
+    return TCT_OK;
+}
+
+extern "C" int tct_interior_pairs_sinh(int kernel, int n_t, const double *tq, const double *tw,
+                                       int n_r, const double *rq, const double *rw,
+                                       const double *obs_pts, const double *obs_ns,
+                                       const double *src_pts, const int32_t *src_tris,
+                                       int64_t n_pairs, const int32_t *pairs,
+                                       const double *params_host, double *out, void *stream) {
+    TCT_CHECK_ARG(kernel >= TCT_U && kernel <= TCT_RT,
+                  "interior_pairs_sinh: kernel %d has no point-observer form", kernel);
+    TCT_CHECK_ARG(n_t > 0 && n_r > 0 && (int64_t)n_t * n_r < (1 << 24),
+                  "interior_pairs_sinh: n_t %d n_r %d", n_t, n_r);
+    TCT_CHECK_ARG(n_pairs >= 0 && n_pairs < (1LL << 26), "interior_pairs_sinh: n_pairs %lld",
+                  (long long)n_pairs);
+    TCT_CHECK_ARG(tq && tw && rq && rw && obs_pts && obs_ns && src_pts && src_tris &&
+                      params_host && out && (pairs || n_pairs == 0),
+                  "interior_pairs_sinh: null pointer");
+    if (n_pairs == 0) return TCT_OK;
+    const ElasticConsts c = elastic_consts(params_host);
+    const dim3 grid((unsigned)ceil_div(n_pairs, 4)), block(256);
+    hipStream_t st = as_stream(stream);
+#define TCT_SK(KID)                                                                            \
+    hipLaunchKernelGGL((interior_pairs_sinh_kernel<KID>), grid, block, 0, st, n_t, tq, tw, n_r, \
+                       rq, rw, obs_pts, obs_ns, src_pts, src_tris, (int)n_pairs, pairs, c, out)
+    switch (kernel) {
+    case TCT_U: TCT_SK(TCT_U); break;
+    case TCT_T: TCT_SK(TCT_T); break;
+    case TCT_A: TCT_SK(TCT_A); break;
+    case TCT_H: TCT_SK(TCT_H); break;
+    default: TCT_SK(TCT_RT); break;
+    }
+#undef TCT_SK
+    TCT_CHECK_LAUNCH("interior_pairs_sinh");
     return TCT_OK;
 }

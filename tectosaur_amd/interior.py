@@ -83,6 +83,17 @@ def _dev_pairs_quad(kid, pairs_dev, n_pairs, quad, obs_pts, obs_ns, src_pts, src
     return out
 
 
+def _dev_pairs_sinh(kid, pairs_dev, n_pairs, rule01, obs_pts, obs_ns, src_pts, src_tris,
+                    params):
+    out = torch.empty((n_pairs, 27), dtype=torch.float64, device=obs_pts.device)
+    n = rule01[1].shape[0]
+    N.call('tct_interior_pairs_sinh', kid, n, N.ptr(rule01[0]), N.ptr(rule01[1]), n,
+           N.ptr(rule01[0]), N.ptr(rule01[1]), N.ptr(obs_pts), N.ptr(obs_ns), N.ptr(src_pts),
+           N.ptr(src_tris), n_pairs, N.ptr(pairs_dev) if n_pairs else None, params.ctypes.data,
+           N.ptr(out), N.stream_ptr())
+    return out
+
+
 def _dev_corners(pairs_dev, n_pairs, quad, obs_ns, src_pts, src_tris, params):
     out = torch.empty((n_pairs, 27), dtype=torch.float64, device=obs_ns.device)
     N.call('tct_interior_corners', quad[1].shape[0], N.ptr(quad[0]), N.ptr(quad[1]),
@@ -206,10 +217,16 @@ class InteriorOp:
     observation points (tectosaur/interior.py:29-119); same signature and
     decomposition.  ``float_type`` selects the farfield N-body precision;
     the pair blocks are FP64.  ``dot`` takes and returns host arrays,
-    ``dot_dev`` device tensors."""
+    ``dot_dev`` device tensors.
+
+    near_method 'gauss' integrates close pairs with the nq_near triangle
+    rule, as the reference does; 'sinh' with the polar sinh rule of
+    sinh_order x sinh_order points per sub-triangle, accurate for points
+    arbitrarily close to (but off) the surface (no reference
+    counterpart)."""
 
     def __init__(self, obs_pts, obs_ns, src_mesh, K_name, threshold, nq_vertex, nq_far, nq_near,
-                 params, float_type):
+                 params, float_type, near_method='gauss', sinh_order=24):
         K = KERNELS[K_name]
         _pt_kernel_ok(K)
         self.K_name = K_name
@@ -241,6 +258,10 @@ class InteriorOp:
         far_q = _quad_dev(gauss2d_tri(nq_far), dev)
         near_q = _quad_dev(gauss2d_tri(nq_near), dev)
         vert_q = _quad_dev(map_to(gaussxw(nq_vertex), [0.0, np.pi / 2]), dev)
+        if near_method not in ('gauss', 'sinh'):
+            raise ValueError("near_method must be 'gauss' or 'sinh'")
+        sinh_q = _quad_dev(map_to(gaussxw(sinh_order), [0.0, 1.0]), dev)
+        self.near_method = near_method
 
         # one block per close pair, in (point, triangle) order: the near-rule
         # or corner block minus the far-rule block the farfield also added
@@ -263,9 +284,14 @@ class InteriorOp:
             vi = np.nonzero(is_vert)[0]
             if ni.shape[0]:
                 sel = torch.as_tensor(ni, dtype=torch.int64, device=dev)
-                blocks.index_copy_(0, sel, _dev_pairs_quad(
-                    K.kid, trip_d.index_select(0, sel).contiguous(), ni.shape[0], near_q, g_obs,
-                    g_ns, g_pts, g_tris, params_a))
+                sub = trip_d.index_select(0, sel).contiguous()
+                if near_method == 'sinh':
+                    near_b = _dev_pairs_sinh(K.kid, sub, ni.shape[0], sinh_q, g_obs, g_ns, g_pts,
+                                             g_tris, params_a)
+                else:
+                    near_b = _dev_pairs_quad(K.kid, sub, ni.shape[0], near_q, g_obs, g_ns, g_pts,
+                                             g_tris, params_a)
+                blocks.index_copy_(0, sel, near_b)
             if vi.shape[0]:
                 sel = torch.as_tensor(vi, dtype=torch.int64, device=dev)
                 blocks.index_copy_(0, sel, _dev_corners(
