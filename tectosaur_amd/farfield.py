@@ -64,20 +64,18 @@ def group_extent(p):
 
 
 def default_tile_cfg(n_obs_pad, n_src_pad, prec, device=None):
-    """Launch shape when no tuned entry exists: enough workgroups to give
-    every CU several, splitting the source list when the observation set
-    alone is too small."""
+    """Launch shape when no tuned entry exists: about 32 workgroups per CU,
+    the source list split into chunks until the observation blocks reach
+    that count (the shape the tuner picks across the served size range:
+    block 128 below 8k observation points, else 256, one point per lane)."""
     n_cu = 256
     if device is not None and torch.cuda.is_available():
         n_cu = torch.cuda.get_device_properties(device).multi_processor_count
-    block = 256
-    per_lane = 2 if n_obs_pad >= 8 * n_cu * block else 1
+    block = 256 if n_obs_pad >= 8192 else 128
+    per_lane = 1
     obs_blocks = -(-n_obs_pad // (block * per_lane))
     n_tiles = n_src_pad // GROUP
-    target = 8 * n_cu
-    chunks = max(1, min(n_tiles, -(-target // obs_blocks)))
-    # keep at least 8 source tiles per chunk so tile set-up stays negligible
-    chunks = max(1, min(chunks, n_tiles // 8 if n_tiles >= 8 else 1))
+    chunks = max(1, min(n_tiles, -(-32 * n_cu // obs_blocks)))
     return N.TileCfg(block, per_lane, chunks, 0)
 
 
