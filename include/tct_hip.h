@@ -148,17 +148,19 @@ int tct_far_reduce_pts(int64_t n, int64_t n_obs_pad, int chunks, const double *p
 /* Galerkin scatter (tectosaur/matrix_free.cl:94-98): out[9t+3b+d] =
  *   sum_q wj_obs[p] phi_b(q) sum_chunks partial[c][d][p],  p = pt_of_tri[t]*nq + q.
  * pt_of_tri [n_tris] int32: first-point position (divided by nq) of each
- * DOF-space obs triangle in the packed order. */
+ * DOF-space obs triangle in the packed order.  With chunks > 1 the chunks
+ * are first summed in place into chunk 0 (partial is consumed). */
 int tct_far_scatter_tris(int64_t n_tris, int nq, const int32_t *pt_of_tri, const double *qphi,
-                         const double *wj_obs, int64_t n_obs_pad, int chunks,
-                         const double *partial, double *out, void *stream);
+                         const double *wj_obs, int64_t n_obs_pad, int chunks, double *partial,
+                         double *out, void *stream);
 
 /* Galerkin scatter of the 9-moment kernels RA and RH: out[9t+3b+k] from the
  * per-point moments and the observation triangle's basis surface curls
- * tri_curl [n_obs_tris][3][3]. */
+ * tri_curl [n_obs_tris][3][3].  partial is consumed as in
+ * tct_far_scatter_tris. */
 int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const int32_t *pt_of_tri,
                          const double *qphi, const double *wj_obs, const double *tri_curl,
-                         int64_t n_obs_pad, int chunks, const double *partial,
+                         int64_t n_obs_pad, int chunks, double *partial,
                          const double *params_host, double *out, void *stream);
 
 /* ---------------- treecode farfield ----------------------------------------

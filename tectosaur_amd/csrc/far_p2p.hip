@@ -342,6 +342,18 @@ __global__ void reduce_pts_kernel(int64_t n, int64_t n_obs_pad, int chunks,
     }
 }
 
+// partial[0][r] = sum_c partial[c][r] over r < stride (one chunk's length),
+// in fixed chunk order; each thread touches only its own r, so in place is
+// safe.  Lets the Galerkin scatters read one chunk with their scattered,
+// triangle-ordered accesses.
+__global__ void chunk_sum_kernel(int64_t stride, int chunks, double *__restrict__ partial) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= stride) return;
+    double s = partial[r];
+    for (int c = 1; c < chunks; ++c) s += partial[(size_t)c * stride + r];
+    partial[r] = s;
+}
+
 __global__ void scatter_tris_kernel(int64_t n_tris, int nq, const int32_t *__restrict__ pt_of_tri,
                                     const double *__restrict__ qphi,
                                     const double *__restrict__ wj_obs, int64_t n_obs_pad,
@@ -619,17 +631,22 @@ extern "C" int tct_far_reduce_pts(int64_t n, int64_t n_obs_pad, int chunks, cons
     return TCT_OK;
 }
 
+static void sum_chunks(hipStream_t st, int64_t stride, int chunks, double *partial) {
+    hipLaunchKernelGGL(chunk_sum_kernel, dim3((unsigned)ceil_div(stride, 256)), dim3(256), 0, st,
+                       stride, chunks, partial);
+}
+
 extern "C" int tct_far_scatter_tris(int64_t n_tris, int nq, const int32_t *pt_of_tri,
                                     const double *qphi, const double *wj_obs, int64_t n_obs_pad,
-                                    int chunks, const double *partial, double *out,
-                                    void *stream) {
+                                    int chunks, double *partial, double *out, void *stream) {
     TCT_CHECK_ARG(n_tris >= 0 && nq > 0 && chunks >= 1 && n_tris * nq <= n_obs_pad,
                   "scatter_tris: bad sizes");
     TCT_CHECK_ARG(pt_of_tri && qphi && wj_obs && partial && out, "scatter_tris: null pointer");
     if (n_tris == 0) return TCT_OK;
+    hipStream_t st = as_stream(stream);
+    if (chunks > 1) sum_chunks(st, 3 * n_obs_pad, chunks, partial);
     hipLaunchKernelGGL(scatter_tris_kernel, dim3((unsigned)ceil_div(n_tris * 9, 256)), dim3(256),
-                       0, as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, n_obs_pad,
-                       chunks, partial, out);
+                       0, st, n_tris, nq, pt_of_tri, qphi, wj_obs, n_obs_pad, 1, partial, out);
     TCT_CHECK_LAUNCH("scatter_tris");
     return TCT_OK;
 }
@@ -637,8 +654,8 @@ extern "C" int tct_far_scatter_tris(int64_t n_tris, int nq, const int32_t *pt_of
 extern "C" int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const int32_t *pt_of_tri,
                                     const double *qphi, const double *wj_obs,
                                     const double *tri_curl, int64_t n_obs_pad, int chunks,
-                                    const double *partial, const double *params_host,
-                                    double *out, void *stream) {
+                                    double *partial, const double *params_host, double *out,
+                                    void *stream) {
     TCT_CHECK_ARG(kernel == TCT_RA || kernel == TCT_RH, "scatter_curl: kernel %d not RA/RH",
                   kernel);
     TCT_CHECK_ARG(n_tris >= 0 && nq > 0 && chunks >= 1 && n_tris * nq <= n_obs_pad,
@@ -648,14 +665,14 @@ extern "C" int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const in
     if (n_tris == 0) return TCT_OK;
     const ElasticConsts c = elastic_consts(params_host);
     const unsigned nb = (unsigned)ceil_div(n_tris, 256);
+    hipStream_t st = as_stream(stream);
+    if (chunks > 1) sum_chunks(st, 9 * n_obs_pad, chunks, partial);
     if (kernel == TCT_RH)
-        hipLaunchKernelGGL(scatter_curl_kernel<TCT_RH>, dim3(nb), dim3(256), 0,
-                           as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl,
-                           n_obs_pad, chunks, partial, c, out);
+        hipLaunchKernelGGL(scatter_curl_kernel<TCT_RH>, dim3(nb), dim3(256), 0, st, n_tris, nq,
+                           pt_of_tri, qphi, wj_obs, tri_curl, n_obs_pad, 1, partial, c, out);
     else
-        hipLaunchKernelGGL(scatter_curl_kernel<TCT_RA>, dim3(nb), dim3(256), 0,
-                           as_stream(stream), n_tris, nq, pt_of_tri, qphi, wj_obs, tri_curl,
-                           n_obs_pad, chunks, partial, c, out);
+        hipLaunchKernelGGL(scatter_curl_kernel<TCT_RA>, dim3(nb), dim3(256), 0, st, n_tris, nq,
+                           pt_of_tri, qphi, wj_obs, tri_curl, n_obs_pad, 1, partial, c, out);
     TCT_CHECK_LAUNCH("scatter_curl");
     return TCT_OK;
 }

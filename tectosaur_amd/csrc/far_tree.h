@@ -200,19 +200,17 @@ __global__ __launch_bounds__(256) void tree_p2m_kernel(int order, int64_t n_src,
     }
 }
 
-// Parent moments from children (one tree level).  Parents are nodes
-// [p_off, p_off + n_par); parent i has children c_off + 2i and, when it
-// exists (< c_off + n_child), c_off + 2i + 1.
+// Moments of node pn from its nc children c0 (and c0 + 1):
 // M_p[a] = sum_child sum_{b <= a} C(a, b) (delta/rho_p)^(a-b) (rho_c/rho_p)^|b| M'_c[b]
 // with delta = c_child - c_parent and M'_c the child moments after the Psi
-// channels absorb the centre shift: Psi' = Psi + delta . q_src.  One block
-// per parent: the children's shifted moments, the binomials and the powers
-// of delta/rho_p and rho_c/rho_p are staged in LDS first.
+// channels absorb the centre shift: Psi' = Psi + delta . q_src.  Called by
+// a whole workgroup: the children's shifted moments, the binomials and the
+// powers of delta/rho_p and rho_c/rho_p are staged in LDS first.  Ends with
+// the node's moments written (no trailing barrier).
 template <int KID, typename T>
-__global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int n_par, int c_off,
-                                                       int n_child,
-                                                       const double *__restrict__ node_cr,
-                                                       T *__restrict__ moments) {
+__device__ __forceinline__ void m2m_node(int order, int pn, int c0, int nc,
+                                         const double *__restrict__ node_cr,
+                                         T *__restrict__ moments) {
     constexpr int C = TreeK<KID>::C;
     constexpr int PM = TCT_TREE_MAX_ORDER + 1;
     constexpr int TMAX = n_terms(TCT_TREE_MAX_ORDER);
@@ -221,12 +219,8 @@ __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int
     __shared__ double pw_d[2][3][PM];
     __shared__ double pw_s[2][PM];
     __shared__ double delta[2][3];
-    const int i = blockIdx.x;
-    if (i >= n_par) return;
     const int TT = n_terms(order);
-    const int pn = p_off + i;
     const double *cp = node_cr + 4 * (size_t)pn;
-    const int nc = (2 * i + 1 < n_child) ? 2 : 1;
     const int tid = threadIdx.x;
     if (tid < PM) {
         double v = 1.0;  // row tid of Pascal's triangle
@@ -237,7 +231,7 @@ __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int
     } else if (tid >= 32 && tid < 32 + 2 * 4) {
         const int cc = (tid - 32) / 4, k = (tid - 32) % 4;
         if (cc < nc) {
-            const double *ccr = node_cr + 4 * (size_t)(c_off + 2 * i + cc);
+            const double *ccr = node_cr + 4 * (size_t)(c0 + cc);
             const double base = k < 3 ? (ccr[k] - cp[k]) / cp[3] : ccr[3] / cp[3];
             double v = 1.0;
             for (int e = 0; e < PM; ++e) {
@@ -252,7 +246,7 @@ __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int
     }
     for (int o = tid; o < nc * C * TT; o += blockDim.x) {
         const int cc = o / (C * TT), r = o % (C * TT);
-        mc[cc][r] = (double)moments[(size_t)(c_off + 2 * i + cc) * C * TT + r];
+        mc[cc][r] = (double)moments[(size_t)(c0 + cc) * C * TT + r];
     }
     __syncthreads();
     // Psi channels absorb the shift of centre (read the unshifted sources)
@@ -282,6 +276,19 @@ __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int
         }
         moments[(size_t)pn * C * TT + o] = (T)acc;
     }
+}
+
+
+// One level of the upward pass: parents p_off.. from children c_off + 2i.
+template <int KID, typename T>
+__global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int n_par, int c_off,
+                                                       int n_child,
+                                                       const double *__restrict__ node_cr,
+                                                       T *__restrict__ moments) {
+    const int i = blockIdx.x;
+    if (i >= n_par) return;
+    m2m_node<KID, T>(order, p_off + i, c_off + 2 * i, (2 * i + 1 < n_child) ? 2 : 1, node_cr,
+                     moments);
 }
 
 // ---------------------------------------------------------------------------

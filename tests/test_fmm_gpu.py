@@ -75,3 +75,4 @@ def test_treecode_all_direct_equals_direct(K, ft, tol):
     y1, y2, op = run(K, 0.0, 2, ft, mac=1e6)
     assert op.tree.stats['m2p'] == 0
     assert np.linalg.norm(y2 - y1) / np.linalg.norm(y1) < tol
+
