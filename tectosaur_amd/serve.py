@@ -83,12 +83,13 @@ def estimate_bytes(n_tris, restart, tree=False):
 
 
 class _Active:
-    def __init__(self, rec, prob, op, state, nbytes):
+    def __init__(self, rec, prob, op, state, nbytes, stream=None):
         self.rec = rec
         self.prob = prob
         self.op = op
         self.state = state
         self.nbytes = nbytes
+        self.stream = stream  # the stream all of this solve's steps run on
         self.busy = False     # a step is in flight
 
 
@@ -135,7 +136,7 @@ def _finalize(a, now, keep_slip):
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
                  hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None,
-                 graphs=True):
+                 graphs=True, streams=4):
         self.mode = mode
         self.max_active = max_active
         self.tol = tol
@@ -152,6 +153,12 @@ class Server:
         # batched servers build new solves on a stream of their own
         # (default priority) so admissions overlap the running solves
         self.build_stream = torch.cuda.Stream() if max_active > 1 else None
+        # batched servers spread the active solves over a few streams
+        # (default priority): one solve's kernels stay in order on its
+        # stream while small solves' kernels overlap the large ones'
+        n_st = streams if max_active > 1 else 0
+        self.streams = [torch.cuda.Stream() for _ in range(n_st)]
+        self._next_stream = 0
         self.ticks = 0
 
     def _build_on_stream(self, prep):
@@ -164,18 +171,38 @@ class Server:
             ev.record()
         return prob, op, state, ev
 
-    @staticmethod
-    def _adopt(active, rec, prob, op, state, need, clock):
+    def _adopt(self, active, rec, prob, op, state, need, clock, ev=None):
         rec.n_tris = prob.n_tris
         rec.pair_evals = prob.pair_evals
         rec.setup_s = clock() - rec.admitted
-        active.append(_Active(rec, prob, op, state, need))
+        stream = None
+        if self.streams:
+            stream = self.streams[self._next_stream % len(self.streams)]
+            self._next_stream += 1
+        if ev is not None:
+            (stream or torch.cuda.current_stream()).wait_event(ev)
+        active.append(_Active(rec, prob, op, state, need, stream))
 
     def _issue(self, batch):
         """Enqueue one step of every solve in ``batch``; their small results
         are gathered on the device and copied to pinned host memory without
         blocking.  Returns (batch, sizes, pinned buffer, event)."""
-        outs = [a.state.issue() for a in batch]
+        main = torch.cuda.current_stream()
+        if self.streams:
+            outs = []
+            used = {}
+            for a in batch:
+                with torch.cuda.stream(a.stream):
+                    outs.append(a.state.issue())
+                used[id(a.stream)] = a.stream
+            for st in used.values():
+                main.wait_stream(st)
+            for o in outs:
+                # read on this stream: keep the allocator from handing the
+                # block to the solve stream's next step before the gather
+                o.record_stream(main)
+        else:
+            outs = [a.state.issue() for a in batch]
         for a in batch:
             a.busy = True
         dev = torch.cat(outs)
@@ -267,14 +294,17 @@ class Server:
                     else:
                         prob, op, state = _build(prep, self.mode, self.tol, self.restart,
                                                  self.maxiter, self.graphs)
-                        self._adopt(active, rec, prob, op, state, need, clock)
+                        ev = None
+                        if self.streams:
+                            ev = torch.cuda.Event()
+                            ev.record()
+                        self._adopt(active, rec, prob, op, state, need, clock, ev)
                     now = clock()
                 # adopt finished builds (block only when nothing else can run)
                 while building and (building[0][0].done() or not active):
                     fut, rec, need = building.popleft()
                     prob, op, state, ev = fut.result()
-                    torch.cuda.current_stream().wait_event(ev)
-                    self._adopt(active, rec, prob, op, state, need, clock)
+                    self._adopt(active, rec, prob, op, state, need, clock, ev)
                 if not active:
                     if qi < len(pending):
                         time.sleep(max(0.0, min(0.002, pending[qi].arrival - clock())))
@@ -301,11 +331,15 @@ class Server:
                 now = clock()
                 finished = set()
                 for a, sz in zip(members, sizes):
-                    done = a.state.consume(host[off:off + sz])
+                    # the host step may enqueue device work (basis vector,
+                    # solution update): on the solve's own stream
+                    with torch.cuda.stream(a.stream or torch.cuda.current_stream()):
+                        done = a.state.consume(host[off:off + sz])
+                        if done:
+                            _finalize(a, now, self.keep_slip)
                     off += sz
                     a.busy = False
                     if done:
-                        _finalize(a, now, self.keep_slip)
                         used -= a.nbytes
                         finished.add(id(a))
                         if log:
