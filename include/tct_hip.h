@@ -318,6 +318,20 @@ int tct_bsr9_spmv_f32(int64_t n_block_rows, const int32_t *indptr, const int32_t
 int tct_csr_spmv(int64_t n_rows, const int32_t *indptr, const int32_t *indices,
                  const double *data, const double *x, double *y, int accumulate, void *stream);
 
+/* ---------------- Krylov basis (device GMRES) -----------------------------
+ * The Arnoldi step of the GMRES the reference calls on the host
+ * (scipy.sparse.linalg.gmres from tectosaur/simple_solver.py:14-34): CGS2
+ * against V [m][n] (row stride ld >= n), every reduction in a fixed order
+ * and no library workspace (solves on different streams share nothing).
+ * tct_krylov_scratch: doubles of scratch for basis size up to m.
+ * tct_cgs2: w <- w - V^T h1, then again with h2; h [m + 1] = (h1 + h2, |w|).
+ * tct_krylov_combine: out <- beta out + alpha V[:m]^T c. */
+int tct_krylov_scratch(int m, int64_t n, int64_t *n_doubles);
+int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w, double *h, double *scratch,
+             int64_t scratch_doubles, void *stream);
+int tct_krylov_combine(int m, int64_t n, int64_t ld, const double *V, const double *c,
+                       double alpha, double beta, double *out, void *stream);
+
 /* ---------------- host-side mesh utilities (CPU, OpenMP) -----------------
  * Reference: fast_find_nearfield.get_nearfield + split_adjacent_close
  * (tectosaur/mesh/fast_find_nearfield.cpp:22-217).

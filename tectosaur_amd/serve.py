@@ -110,14 +110,14 @@ def _prepare(req, orders, tree=None):
     return prep
 
 
-def _build(prep, mode, tol, restart, maxiter, graphs=False):
+def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
     prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
     b = op.cmT.dot_dev(prob.rhs)
     state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
     if graphs:
-        op.enable_graphs(MODES[mode][0:2] + MODES[mode][3:4])
+        op.enable_graphs(MODES[mode][0:2] + MODES[mode][3:4], stream=capture_stream)
     return prob, op, state
 
 
@@ -158,6 +158,9 @@ class Server:
         # stream while small solves' kernels overlap the large ones'
         n_st = streams if max_active > 1 else 0
         self.streams = [torch.cuda.Stream() for _ in range(n_st)]
+        # every graph capture of this server runs on this one stream (no
+        # further pooled streams are taken while solves run)
+        self.capture_stream = torch.cuda.Stream() if graphs else None
         self._next_stream = 0
         self.ticks = 0
 
@@ -166,7 +169,7 @@ class Server:
         returns its objects and an event the compute stream waits on."""
         with torch.cuda.stream(self.build_stream):
             prob, op, state = _build(prep, self.mode, self.tol, self.restart, self.maxiter,
-                                     self.graphs)
+                                     self.graphs, self.capture_stream)
             ev = torch.cuda.Event()
             ev.record()
         return prob, op, state, ev
@@ -293,7 +296,7 @@ class Server:
                                          need))
                     else:
                         prob, op, state = _build(prep, self.mode, self.tol, self.restart,
-                                                 self.maxiter, self.graphs)
+                                                 self.maxiter, self.graphs, self.capture_stream)
                         ev = None
                         if self.streams:
                             ev = torch.cuda.Event()

@@ -22,6 +22,7 @@ the Krylov vectors, the operator and the orthogonalisation stay in HBM:
 
 Tolerances are relative to ||rhs|| of the constrained system, as scipy's.
 """
+import ctypes
 import math
 import time
 
@@ -50,7 +51,7 @@ class ConstrainedOperator:
         self.A.dot_dev(self._u, self._y, prec)
         self.cmT.dot_dev(self._y, out)
 
-    def enable_graphs(self, precs):
+    def enable_graphs(self, precs, stream=None):
         """Capture the whole apply (CSR, farfield passes, BSR, CSR) for each
         precision in ``precs`` into a HIP graph replayed by ``apply``: one
         graph launch and two vector copies per application instead of a
@@ -59,12 +60,15 @@ class ConstrainedOperator:
 
         Capture runs on a side stream ordered after the current one, without
         a device-wide synchronisation, so a server can build a new solve's
-        graphs while other solves' work is in flight."""
+        graphs while other solves' work is in flight.  A server passes its
+        one capture stream: torch hands out pooled streams round-robin, so a
+        fresh stream per solve would in time alias a stream other solves
+        replay on."""
         n = self.shape[0]
         self._gx = torch.zeros(n, dtype=torch.float64, device=self.device)
         self._gy = torch.empty(n, dtype=torch.float64, device=self.device)
         cur = torch.cuda.current_stream(self.device)
-        side = torch.cuda.Stream(self.device)
+        side = stream if stream is not None else torch.cuda.Stream(self.device)
         for prec in precs:
             if prec is None or prec in self._graphs:
                 continue
@@ -130,7 +134,7 @@ class GmresState:
 
     def __init__(self, apply, b, tol=1e-8, restart=200, maxiter=2000, inner_prec='f64',
                  outer_prec='f64', inner_reduction=None, x0=None, callback=None,
-                 coarse_prec=None, coarse_above=3e-5):
+                 coarse_prec=None, coarse_above=3e-5, update_prec=None, update_below=1e-3):
         self.apply = apply
         self.b = b
         dev = b.device
@@ -147,6 +151,16 @@ class GmresState:
         # convergence is only ever declared from an outer_prec residual
         self.coarse_prec = coarse_prec
         self.coarse_above = coarse_above
+        # once an outer_prec residual is below update_below * ||b||, later
+        # residuals may be updated, r <- r - A d with d the cycle's
+        # correction, in update_prec: the update's error is relative to
+        # |A d| ~ |r|, so a cheap operator keeps the residual accurate to far
+        # below the target; a residual that meets the target this way is
+        # confirmed with an outer_prec one before convergence is declared
+        self.update_prec = update_prec
+        self.update_below = update_below
+        self._anchored = False    # the residual chain starts from an outer_prec one
+        self._confirm = False     # the next residual must be outer_prec
         self._expect = float('inf')
         self._res_prec = outer_prec
         self.callback = callback
@@ -155,6 +169,17 @@ class GmresState:
         self.V = torch.empty((self.m + 1, n), dtype=torch.float64, device=dev)
         self.w = torch.empty(n, dtype=torch.float64, device=dev)
         self.r = torch.empty(n, dtype=torch.float64, device=dev)
+        self.d = torch.zeros(n, dtype=torch.float64, device=dev) if update_prec else None
+        # device solves orthogonalise with the workspace-free HIP kernels of
+        # krylov.hip (fixed reduction order, nothing shared between solves
+        # on different streams); host tensors use torch
+        self._hip = dev.type == 'cuda'
+        if self._hip:
+            from . import _native as N
+            nd = ctypes.c_int64(0)
+            N.call('tct_krylov_scratch', self.m + 1, n, ctypes.byref(nd))
+            self._scratch = torch.empty(nd.value, dtype=torch.float64, device=dev)
+            self._h = torch.empty(self.m + 2, dtype=torch.float64, device=dev)
         self.it = 0
         self.restarts = 0
         self.residuals = []
@@ -166,7 +191,7 @@ class GmresState:
 
     @property
     def device_bytes(self):
-        return (self.m + 4) * self.n * 8
+        return (self.m + 5) * self.n * 8
 
     # -- device side --------------------------------------------------------
     def issue(self):
@@ -175,11 +200,20 @@ class GmresState:
         if self.phase == 'norm_b':
             return torch.linalg.vector_norm(self.b).reshape(1)
         if self.phase == 'residual':
+            self._res_kind = 'full'
             if self._fresh and self.restarts == 0:
                 self.r.copy_(self.b)
                 self._res_prec = self.outer_prec
+            elif (self.update_prec is not None and self._anchored and not self._confirm and
+                  self.rel <= self.update_below):
+                # r <- r - A d (w is free between cycles)
+                self._res_prec = self.update_prec
+                self._res_kind = 'update'
+                self.apply(self.d, self.w, self.update_prec)
+                self.r.sub_(self.w)
             else:
-                coarse = self.coarse_prec is not None and self._expect > self.coarse_above
+                coarse = (self.coarse_prec is not None and not self._confirm and
+                          self._expect > self.coarse_above)
                 self._res_prec = self.coarse_prec if coarse else self.outer_prec
                 self.apply(self.x, self.r, self._res_prec)
                 torch.sub(self.b, self.r, out=self.r)
@@ -187,6 +221,12 @@ class GmresState:
         # arnoldi step j
         j = self.j
         self.apply(self.V[j], self.w, self.inner_prec)
+        if self._hip:
+            from . import _native as N
+            N.call('tct_cgs2', j + 1, self.n, self.n, N.ptr(self.V), N.ptr(self.w),
+                   N.ptr(self._h), N.ptr(self._scratch), self._scratch.shape[0],
+                   N.stream_ptr())
+            return self._h[:j + 2]
         Vj = self.V[:j + 1]
         h1 = torch.mv(Vj, self.w)
         self.w.sub_(torch.mv(Vj.t(), h1))
@@ -216,10 +256,17 @@ class GmresState:
                 self.callback(self.rel)
             if not math.isfinite(beta):
                 return self._finish('nonfinite')
+            exact = self._res_prec == self.outer_prec and self._res_kind == 'full'
+            if exact:
+                self._anchored = True
+            elif self._res_kind == 'full':
+                self._anchored = False
+            self._confirm = False
             if beta <= self.target:
-                if self._res_prec != self.outer_prec:
-                    # a coarse residual never decides convergence
+                if not exact:
+                    # a coarse or updated residual never decides convergence
                     self._expect = 0.0
+                    self._confirm = True
                     return False
                 return self._finish('converged')
             if self.it >= self.maxiter:
@@ -267,8 +314,23 @@ class GmresState:
         if math.isfinite(est):
             y = np.linalg.solve(np.triu(self.H[:k, :k]), self.g[:k])
             if np.all(np.isfinite(y)):
-                self.x.add_(torch.mv(self.V[:k].t(),
-                                     torch.as_tensor(y, dtype=torch.float64, device=self.x.device)))
+                y_t = torch.as_tensor(y, dtype=torch.float64, device=self.x.device)
+                if self._hip:
+                    from . import _native as N
+                    tgt, beta = (self.d, 0.0) if self.d is not None else (self.x, 1.0)
+                    N.call('tct_krylov_combine', k, self.n, self.n, N.ptr(self.V), N.ptr(y_t),
+                           1.0, beta, N.ptr(tgt), N.stream_ptr())
+                    if self.d is not None:
+                        self.x.add_(self.d)
+                elif self.d is not None:
+                    torch.mv(self.V[:k].t(), y_t, out=self.d)
+                    self.x.add_(self.d)
+                else:
+                    self.x.add_(torch.mv(self.V[:k].t(), y_t))
+            elif self.d is not None:
+                self.d.zero_()
+        elif self.d is not None:
+            self.d.zero_()
         self.phase = 'residual'
         return False
 
@@ -306,6 +368,10 @@ MODES = {
     # what one cycle can gain
     'tree': ('tree', 'f64', 3e-3, 'f32'),
 }
+# residual updates r <- r - A d in FP32 (direct farfield, FP32 nearfield
+# blocks) once the FP64 residual is below 1e-3 ||b||: FP32's ~1e-6 relative
+# error on A d stays far below the 1e-8 target
+UPDATE_PREC = {'f64': None, 'mixed': 'f32', 'tree': 'f32'}
 
 
 def make_state(op, b, mode='mixed', tol=1e-8, restart=200, maxiter=2000, callback=None,
@@ -313,7 +379,7 @@ def make_state(op, b, mode='mixed', tol=1e-8, restart=200, maxiter=2000, callbac
     inner, outer, red, coarse = MODES[mode]
     return GmresState(op.apply, b, tol=tol, restart=restart, maxiter=maxiter, inner_prec=inner,
                       outer_prec=outer, inner_reduction=inner_reduction or red, callback=callback,
-                      coarse_prec=coarse)
+                      coarse_prec=coarse, update_prec=UPDATE_PREC[mode])
 
 
 def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxiter=6000,

@@ -75,6 +75,54 @@ def test_coarse_residuals_never_decide_convergence():
     assert np.linalg.norm(A @ res.x.numpy() - b.numpy()) <= 1e-10 * np.linalg.norm(b.numpy())
 
 
+def test_residual_updates_save_exact_residuals_and_never_decide():
+    # as above, plus residual updates r <- r - A_f32 d once the exact
+    # residual is small: fewer exact ('f64') applications, the same
+    # converged accuracy, and the deciding residual is still exact
+    from tectosaur_amd.solver import GmresState
+    rng = np.random.default_rng(4)
+    n = 120
+    A = np.eye(n) * 3 + rng.normal(size=(n, n)) / np.sqrt(n)
+    ops = {'f64': torch.as_tensor(A),
+           'f32': torch.as_tensor(A + 1e-7 * rng.normal(size=(n, n)) / np.sqrt(n)),
+           'tree': torch.as_tensor(A + 3e-3 * rng.normal(size=(n, n)) / np.sqrt(n))}
+    b = torch.as_tensor(rng.normal(size=n))
+    counts = {}
+    for upd in (None, 'f32'):
+        used = []
+
+        def apply(x, out, prec):
+            used.append(prec)
+            torch.mv(ops[prec], x, out=out)
+            return out
+        st = GmresState(apply, b, tol=1e-10, restart=60, maxiter=4000, inner_prec='tree',
+                        outer_prec='f64', inner_reduction=3e-3, coarse_prec='f32',
+                        coarse_above=3e-5, update_prec=upd, update_below=1e-3)
+        res = st.run()
+        assert res.converged
+        assert used[-1] == 'f64'
+        assert np.linalg.norm(A @ res.x.numpy() - b.numpy()) <= 1e-10 * np.linalg.norm(b.numpy())
+        counts[upd] = used.count('f64')
+    assert counts['f32'] < counts[None]
+    # a planted error in the update operator (1e-3 off) would stall a
+    # converged-looking update: the exact confirmation then keeps iterating
+    # rather than accepting it
+    used = []
+    bad = {'f64': ops['f64'], 'tree': ops['tree'],
+           'f32': torch.as_tensor(A + 1e-3 * rng.normal(size=(n, n)) / np.sqrt(n))}
+
+    def apply_bad(x, out, prec):
+        used.append(prec)
+        torch.mv(bad[prec], x, out=out)
+        return out
+    res = GmresState(apply_bad, b, tol=1e-10, restart=60, maxiter=4000, inner_prec='tree',
+                     outer_prec='f64', inner_reduction=3e-3, coarse_prec=None,
+                     update_prec='f32', update_below=1e-3).run()
+    assert used[-1] == 'f64'
+    if res.converged:
+        assert np.linalg.norm(A @ res.x.numpy() - b.numpy()) <= 1e-10 * np.linalg.norm(b.numpy())
+
+
 def test_gmres_zero_rhs_and_nan():
     A = np.eye(4)
     r = gmres(dense_apply(A), torch.zeros(4, dtype=torch.float64), tol=1e-8)
