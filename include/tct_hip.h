@@ -324,11 +324,13 @@ int tct_csr_spmv(int64_t n_rows, const int32_t *indptr, const int32_t *indices,
  * against V [m][n] (row stride ld >= n), every reduction in a fixed order
  * and no library workspace (solves on different streams share nothing).
  * tct_krylov_scratch: doubles of scratch for basis size up to m.
- * tct_cgs2: w <- w - V^T h1, then again with h2; h [m + 1] = (h1 + h2, |w|).
+ * tct_cgs2: w <- w - V^T h1, then again with h2; h [m + 1] = (h1 + h2, |w|);
+ *   vnext (and vnext2), when not NULL, receive w / |w| (the next basis
+ *   vector, without a host round trip).
  * tct_krylov_combine: out <- beta out + alpha V[:m]^T c. */
 int tct_krylov_scratch(int m, int64_t n, int64_t *n_doubles);
 int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w, double *h, double *scratch,
-             int64_t scratch_doubles, void *stream);
+             int64_t scratch_doubles, double *vnext, double *vnext2, void *stream);
 int tct_krylov_combine(int m, int64_t n, int64_t ld, const double *V, const double *c,
                        double alpha, double beta, double *out, void *stream);
 

@@ -68,7 +68,7 @@ SIGNATURES = {
     'tct_bsr39_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_limit_combine': [_int, _i64, _int, _vp, _vp, _vp, _vp],
     'tct_krylov_scratch': [_int, _i64, _vp],
-    'tct_cgs2': [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp],
+    'tct_cgs2': [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     'tct_krylov_combine': [_int, _i64, _i64, _vp, _vp, _dbl, _dbl, _vp, _vp],
     'tct_interior_pairs_sinh': [_int, _int, _vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                                 _vp, _vp, _vp, _vp],

@@ -75,6 +75,19 @@ __global__ __launch_bounds__(KB) void combine_kernel(int m, int64_t n, int64_t l
     out[t] = (beta == 0.0 ? 0.0 : beta * out[t]) + alpha * s;
 }
 
+// vnext[t] = w[t] / *norm (and vnext2 likewise): the next basis vector,
+// formed on the device from the norm the step just computed.
+__global__ __launch_bounds__(KB) void normalize_kernel(int64_t n, const double *__restrict__ w,
+                                                       const double *__restrict__ norm,
+                                                       double *__restrict__ vnext,
+                                                       double *__restrict__ vnext2) {
+    const int64_t t = (int64_t)blockIdx.x * KB + threadIdx.x;
+    if (t >= n) return;
+    const double v = w[t] / *norm;
+    vnext[t] = v;
+    if (vnext2) vnext2[t] = v;
+}
+
 inline int n_blocks(int64_t n) {
     int64_t nb = (n + 4 * KB - 1) / (4 * KB);
     return (int)(nb < 1 ? 1 : (nb > 1024 ? 1024 : nb));
@@ -92,7 +105,8 @@ extern "C" int tct_krylov_scratch(int m, int64_t n, int64_t *n_doubles) {
 }
 
 extern "C" int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w, double *h,
-                        double *scratch, int64_t scratch_doubles, void *stream) {
+                        double *scratch, int64_t scratch_doubles, double *vnext, double *vnext2,
+                        void *stream) {
     TCT_CHECK_ARG(m >= 1 && n >= 1 && ld >= n && m < (1 << 20), "cgs2: m %d n %lld ld %lld", m,
                   (long long)n, (long long)ld);
     TCT_CHECK_ARG(V && w && h && scratch, "cgs2: null pointer");
@@ -118,6 +132,9 @@ extern "C" int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w
     hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, 0, n, ld, V, w, 1, partial);
     hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(64), 0, st, 1, nb, partial,
                        (const double *)nullptr, 1, h + m);
+    if (vnext)
+        hipLaunchKernelGGL(normalize_kernel, dim3(gp), dim3(KB), 0, st, n, w, h + m, vnext,
+                           vnext2);
     TCT_CHECK_LAUNCH("cgs2");
     return TCT_OK;
 }

@@ -68,9 +68,9 @@ def default_tile_cfg(n_obs_pad, n_src_pad, prec, device=None):
     the source list split into chunks until the observation blocks reach
     that count (the shape the tuner picks across the served size range:
     block 128 below 8k observation points, else 256, one point per lane)."""
+    # a fixed CU count (MI355X: 256), not the device's: the chunk count
+    # sets the summation order, so results do not depend on the box
     n_cu = 256
-    if device is not None and torch.cuda.is_available():
-        n_cu = torch.cuda.get_device_properties(device).multi_processor_count
     block = 256 if n_obs_pad >= 8192 else 128
     per_lane = 1
     obs_blocks = -(-n_obs_pad // (block * per_lane))

@@ -214,9 +214,10 @@ class TreeFarfield:
         self.C = N_CHANNELS[engine.K.name]
         self.T = n_terms(order)
         if n_waves is None:
-            # enough wavefronts to hide the latency of each wave's serial list
-            n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
-            n_waves = 64 * n_cu
+            # enough wavefronts to hide the latency of each wave's serial
+            # list: 64 per CU of an MI355X (256 CUs), fixed rather than read
+            # from the device since the segmentation sets the summation order
+            n_waves = 64 * 256
         seg, seg_off, self.seg_len = segments(start, n_waves)
         self.n_seg = seg.shape[0]
         self.seg = torch.as_tensor(seg, device=dev)

@@ -28,6 +28,7 @@ default) the batched path is compared with.
 """
 import collections
 import concurrent.futures
+import sys
 import time
 
 import numpy as np
@@ -136,7 +137,7 @@ def _finalize(a, now, keep_slip):
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
                  hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None,
-                 graphs=True, streams=4):
+                 graphs=True, streams=4, switch_interval=1e-3):
         self.mode = mode
         self.max_active = max_active
         self.tol = tol
@@ -148,6 +149,11 @@ class Server:
         self.tree = (tree or dict(DEFAULT_TREE)) if mode == 'tree' else None
         self.keep_slip = keep_slip
         self.prep_workers = prep_workers
+        # the loop, the builder thread and the preparation threads share the
+        # GIL: a short switch interval keeps the ticking loop responsive
+        # (preparation in worker processes measured 2x slower: its results
+        # are large and must be pickled back)
+        self.switch_interval = switch_interval
         # capture each solve's operator applications into HIP graphs
         self.graphs = graphs
         # batched servers build new solves on a stream of their own
@@ -235,6 +241,9 @@ class Server:
         budget = self.hbm_fraction * free0
         pool = (concurrent.futures.ThreadPoolExecutor(self.prep_workers)
                 if self.prep_workers > 0 else None)
+        old_switch = sys.getswitchinterval()
+        if self.switch_interval:
+            sys.setswitchinterval(self.switch_interval)
         futures = {}
         max_ahead = self.max_active + 2 * max(self.prep_workers, 1)
         t0 = time.perf_counter()
@@ -354,6 +363,7 @@ class Server:
                 if finished:
                     active = [a for a in active if id(a) not in finished]
         finally:
+            sys.setswitchinterval(old_switch)
             if pool is not None:
                 pool.shutdown(wait=True, cancel_futures=True)
             if builder is not None:

@@ -86,6 +86,23 @@ class ConstrainedOperator:
             cur.wait_stream(side)
             self._graphs[prec] = g
 
+    def has_graph(self, prec):
+        return prec in self._graphs
+
+    def replay(self, prec):
+        """One captured application: graph_input -> graph_output (the
+        buffers ``gx``/``gy``), with no copies around it."""
+        self._graphs[prec].replay()
+        self.n_matvec[prec] += 1
+
+    @property
+    def gx(self):
+        return self._gx
+
+    @property
+    def gy(self):
+        return self._gy
+
     def apply(self, x, out=None, prec='f64'):
         if out is None:
             out = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
@@ -134,7 +151,8 @@ class GmresState:
 
     def __init__(self, apply, b, tol=1e-8, restart=200, maxiter=2000, inner_prec='f64',
                  outer_prec='f64', inner_reduction=None, x0=None, callback=None,
-                 coarse_prec=None, coarse_above=3e-5, update_prec=None, update_below=1e-3):
+                 coarse_prec=None, coarse_above=3e-5, update_prec=None, update_below=1e-3,
+                 operator=None):
         self.apply = apply
         self.b = b
         dev = b.device
@@ -174,6 +192,12 @@ class GmresState:
         # krylov.hip (fixed reduction order, nothing shared between solves
         # on different streams); host tensors use torch
         self._hip = dev.type == 'cuda'
+        # with the operator's inner-precision graph, an Arnoldi step is one
+        # replay reading the graph input buffer and writing the graph output,
+        # which is orthogonalised in place; the next basis vector goes
+        # straight into V and the graph input (no copies, no host step)
+        self._op = operator
+        self._fast = False
         if self._hip:
             from . import _native as N
             nd = ctypes.c_int64(0)
@@ -220,11 +244,19 @@ class GmresState:
             return torch.linalg.vector_norm(self.r).reshape(1)
         # arnoldi step j
         j = self.j
+        if self._fast:
+            from . import _native as N
+            op = self._op
+            op.replay(self.inner_prec)
+            N.call('tct_cgs2', j + 1, self.n, self.n, N.ptr(self.V), N.ptr(op.gy),
+                   N.ptr(self._h), N.ptr(self._scratch), self._scratch.shape[0],
+                   N.ptr(self.V[j + 1]), N.ptr(op.gx), N.stream_ptr())
+            return self._h[:j + 2]
         self.apply(self.V[j], self.w, self.inner_prec)
         if self._hip:
             from . import _native as N
             N.call('tct_cgs2', j + 1, self.n, self.n, N.ptr(self.V), N.ptr(self.w),
-                   N.ptr(self._h), N.ptr(self._scratch), self._scratch.shape[0],
+                   N.ptr(self._h), N.ptr(self._scratch), self._scratch.shape[0], None, None,
                    N.stream_ptr())
             return self._h[:j + 2]
         Vj = self.V[:j + 1]
@@ -273,12 +305,16 @@ class GmresState:
                 return self._finish('maxiter')
             self.restarts += 1
             self.V[0].copy_(self.r).div_(beta)
-            m = self.m
-            self.H = np.zeros((m + 1, m))
-            self.g = np.zeros(m + 1)
-            self.g[0] = beta
-            self.cs = np.zeros(m)
-            self.sn = np.zeros(m)
+            self._fast = (self._hip and self._op is not None and
+                          self._op.has_graph(self.inner_prec))
+            if self._fast:
+                self._op.gx.copy_(self.V[0])
+            # Givens state in Python floats (the per-step host work is a few
+            # dozen scalar operations); R's columns kept as lists
+            self.Rcols = []
+            self.g = [beta]
+            self.cs = []
+            self.sn = []
             self.cycle_target = self.target
             if self.inner_reduction is not None:
                 self.cycle_target = max(self.target * 0.5, self.inner_reduction * beta)
@@ -287,32 +323,38 @@ class GmresState:
             return False
         # arnoldi
         j = self.j
-        col = vals.copy()
-        hnext = float(col[j + 1])
+        col = vals.tolist()
+        hnext = col[j + 1]
         self.it += 1
         cs, sn = self.cs, self.sn
         for i in range(j):
-            t = cs[i] * col[i] + sn[i] * col[i + 1]
-            col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1]
-            col[i] = t
-        cs[j], sn[j] = _givens(col[j], col[j + 1])
-        col[j] = cs[j] * col[j] + sn[j] * col[j + 1]
-        col[j + 1] = 0.0
-        self.H[:j + 2, j] = col
-        self.g[j + 1] = -sn[j] * self.g[j]
-        self.g[j] = cs[j] * self.g[j]
+            a, b = col[i], col[i + 1]
+            col[i] = cs[i] * a + sn[i] * b
+            col[i + 1] = -sn[i] * a + cs[i] * b
+        c, sv = _givens(col[j], col[j + 1])
+        cs.append(c)
+        sn.append(sv)
+        col[j] = c * col[j] + sv * col[j + 1]
+        self.Rcols.append(col[:j + 1])
+        gj = self.g[j]
+        self.g[j] = c * gj
+        self.g.append(-sv * gj)
         est = abs(self.g[j + 1])
         k = j + 1
         end = (not math.isfinite(est) or est <= self.cycle_target or hnext == 0.0 or
                k >= self.m or self.it >= self.maxiter)
         if not end:
-            self.V[k].copy_(self.w).div_(hnext)
+            if not self._fast:
+                self.V[k].copy_(self.w).div_(hnext)
             self.j = k
             return False
         # close the cycle: x += V_k y
         self._expect = est / self.bnorm if math.isfinite(est) else float('inf')
         if math.isfinite(est):
-            y = np.linalg.solve(np.triu(self.H[:k, :k]), self.g[:k])
+            R = np.zeros((k, k))
+            for jj, cj in enumerate(self.Rcols):
+                R[:jj + 1, jj] = cj
+            y = np.linalg.solve(R, np.asarray(self.g[:k]))
             if np.all(np.isfinite(y)):
                 y_t = torch.as_tensor(y, dtype=torch.float64, device=self.x.device)
                 if self._hip:
@@ -379,7 +421,7 @@ def make_state(op, b, mode='mixed', tol=1e-8, restart=200, maxiter=2000, callbac
     inner, outer, red, coarse = MODES[mode]
     return GmresState(op.apply, b, tol=tol, restart=restart, maxiter=maxiter, inner_prec=inner,
                       outer_prec=outer, inner_reduction=inner_reduction or red, callback=callback,
-                      coarse_prec=coarse, update_prec=UPDATE_PREC[mode])
+                      coarse_prec=coarse, update_prec=UPDATE_PREC[mode], operator=op)
 
 
 def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxiter=6000,

@@ -21,8 +21,10 @@ def test_cgs2_and_combine_match_torch(m, n):
     scratch = torch.empty(nd.value, dtype=torch.float64, device='cuda')
     h = torch.empty(m + 1, dtype=torch.float64, device='cuda')
     w = w0.clone()
+    vn = torch.empty(n, dtype=torch.float64, device='cuda')
+    vn2 = torch.empty(n, dtype=torch.float64, device='cuda')
     N.call('tct_cgs2', m, n, n, N.ptr(V), N.ptr(w), N.ptr(h), N.ptr(scratch), nd.value,
-           N.stream_ptr())
+           N.ptr(vn), N.ptr(vn2), N.stream_ptr())
     h1 = V @ w0
     wr = w0 - V.t() @ h1
     h2 = V @ wr
@@ -30,11 +32,12 @@ def test_cgs2_and_combine_match_torch(m, n):
     ref_h = torch.cat([h1 + h2, torch.linalg.vector_norm(wr).reshape(1)])
     assert torch.allclose(h, ref_h, rtol=1e-12, atol=1e-12 * float(ref_h.abs().max()))
     assert torch.allclose(w, wr, rtol=0, atol=1e-12 * float(w0.abs().max()))
+    assert torch.equal(vn, w / h[m]) and torch.equal(vn2, vn)
     # repeated calls give bitwise identical results (fixed reduction order)
     w2 = w0.clone()
     h2b = torch.empty_like(h)
     N.call('tct_cgs2', m, n, n, N.ptr(V), N.ptr(w2), N.ptr(h2b), N.ptr(scratch), nd.value,
-           N.stream_ptr())
+           None, None, N.stream_ptr())
     assert torch.equal(h2b, h) and torch.equal(w2, w)
     # combine: out = beta out + alpha V^T c
     c = torch.as_tensor(rng.standard_normal(m), device='cuda')
