@@ -269,6 +269,80 @@ def pairfind_bench(pts, tris, thr=2.5, reps=3):
     return res
 
 
+def large_mesh():
+    """The large-solve mesh: a 40 km free surface of 512 x 512 squares with
+    sinusoidal topography (amplitude 500 m, wavelength 10 km; zero along
+    the fault trace) plus a 60-degree dipping fault of 256 x 256 squares
+    whose top edge lies in the surface: 655,360 triangles."""
+    from tectosaur_amd import mesh
+    pts, tris, n_surf = mesh.fault_surface_mesh(513, 257, 257, surf_w=20000.0,
+                                                fault_L=10000.0, dip_deg=60.0)
+    surf = np.zeros(pts.shape[0], dtype=bool)
+    surf[np.unique(tris[:n_surf])] = True
+    lam = 10000.0
+    pts[surf, 2] += 500.0 * np.sin(2 * np.pi * pts[surf, 0] / lam) * \
+        np.sin(2 * np.pi * pts[surf, 1] / lam)
+    return pts, tris
+
+
+def large_solve_bench(mode='tree', tol=1e-8):
+    """One traction -> slip solve (RH3 near and far, continuity and
+    free-edge constraints) on the 655,360-triangle fault-plus-topography
+    mesh: FP32 treecode farfield in the Arnoldi cycles, FP64 direct
+    residuals; traction (1, 0.5, -0.3) + N(0, 0.1) noise (default_rng(0)).
+    Reports set-up and solve times, iterations, residual evaluations, the
+    FP32/FP64 direct and treecode matvec times and peak device memory."""
+    import torch
+    from tectosaur_amd.solver import solve_constrained
+    from tectosaur_amd.traction_slip import (DEFAULT_TREE, TractionSlipProblem,
+                                             prepare_problem)
+    pts, tris = large_mesh()
+    n = tris.shape[0]
+    t = np.tile([1.0, 0.5, -0.3], 3 * n) + np.random.default_rng(0).normal(0, 0.1, 9 * n)
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    prep = prepare_problem(pts, tris, t, tree=dict(DEFAULT_TREE))
+    t_prep = time.perf_counter() - t0
+    print('# large solve: %d tris, host prep %.1f s' % (n, t_prep), flush=True)
+    t0 = time.perf_counter()
+    prob = TractionSlipProblem(None, None, None, farfield_precisions=('f32', 'f64'),
+                               prepared=prep)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    print('# large solve: device build %.1f s' % t_build, flush=True)
+    ff = prob.H.farfield
+    v = torch.as_tensor(np.random.default_rng(1).standard_normal(9 * n), device='cuda')
+    mv = {}
+    for prec in ('tree', 'f32', 'f64'):
+        ff.dot_dev(v, prec=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ff.dot_dev(v, prec=prec)
+        torch.cuda.synchronize()
+        mv[prec + '_farfield_ms'] = 1e3 * (time.perf_counter() - t0)
+        print('# large solve: %s farfield matvec %.1f ms' % (prec, mv[prec + '_farfield_ms']),
+              flush=True)
+    t0 = time.perf_counter()
+
+    def progress(rel):
+        print('# large solve: residual %.3e at %.1f s' % (rel, time.perf_counter() - t0),
+              flush=True)
+    u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=tol, mode=mode, callback=progress)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t0
+    out = {'workload': 'traction -> slip, %d-tri fault + topography mesh, RH3 near/far, '
+                       'tol %g, mode %s' % (n, tol, mode),
+           'n_tris': int(n), 'n_dofs_free': int(prob.n_dofs), 'host_prep_s': t_prep,
+           'device_build_s': t_build, 'solve_s': t_solve, 'converged': bool(res.converged),
+           'iterations': int(res.iterations), 'restarts': int(res.restarts),
+           'rel_residual_fp64': float(res.rel_residual), 'matvecs': res.n_matvec,
+           'pair_evals_per_direct_matvec': int(prob.pair_evals),
+           'peak_device_GB': torch.cuda.max_memory_allocated() / 1e9}
+    out.update(mv)
+    print('# large solve: %s' % json.dumps(out), flush=True)
+    return out
+
+
 def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
     """The CPU oracle (C restatement of farfield_tris_to_tris, FP64 with
     long-double accumulation, OpenMP) on a bounded sample of observation
@@ -345,6 +419,8 @@ def main():
     ap.add_argument('--no-treecode', action='store_true')
     ap.add_argument('--no-interior', action='store_true')
     ap.add_argument('--no-pairfind', action='store_true')
+    ap.add_argument('--large-solve', action='store_true',
+                    help='only the single large solve (655,360 triangles; minutes)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run a 1 GiB device copy first (scale for byte counters)')
     args = ap.parse_args()
@@ -356,6 +432,9 @@ def main():
     import torch
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
     torch.cuda.set_device(0)
+    if args.large_solve:
+        print(json.dumps({'large_solve': large_solve_bench()}), flush=True)
+        return
     calib = copy_calibration() if args.copy_calibration else None
     t_setup = time.perf_counter()
     pts, tris = cfg2_mesh()
