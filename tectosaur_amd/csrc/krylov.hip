@@ -47,22 +47,6 @@ __global__ __launch_bounds__(KB) void multi_dot_kernel(int m, int64_t n, int64_t
     }
 }
 
-// h[i] (+)= sum_b partial[b][i] for i < cols; with sqrt_last the last
-// column is a sum of squares and its square root is written.
-__global__ void reduce_cols_kernel(int cols, int nb, const double *__restrict__ partial,
-                                   const double *__restrict__ add, int sqrt_last,
-                                   double *__restrict__ h) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cols) return;
-    double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += partial[(size_t)b * cols + i];
-    if (sqrt_last && i == cols - 1)
-        s = sqrt(s);
-    else if (add)
-        s += add[i];
-    h[i] = s;
-}
-
 // out[t] = beta out[t] + alpha sum_i V[i][t] c[i]   (one thread per point)
 __global__ __launch_bounds__(KB) void combine_kernel(int m, int64_t n, int64_t ld,
                                                      const double *__restrict__ V,
@@ -75,17 +59,58 @@ __global__ __launch_bounds__(KB) void combine_kernel(int m, int64_t n, int64_t l
     out[t] = (beta == 0.0 ? 0.0 : beta * out[t]) + alpha * s;
 }
 
-// vnext[t] = w[t] / *norm (and vnext2 likewise): the next basis vector,
-// formed on the device from the norm the step just computed.
-__global__ __launch_bounds__(KB) void normalize_kernel(int64_t n, const double *__restrict__ w,
-                                                       const double *__restrict__ norm,
-                                                       double *__restrict__ vnext,
-                                                       double *__restrict__ vnext2) {
-    const int64_t t = (int64_t)blockIdx.x * KB + threadIdx.x;
-    if (t >= n) return;
-    const double v = w[t] / *norm;
-    vnext[t] = v;
-    if (vnext2) vnext2[t] = v;
+// The reduce and the update in one launch: every block first sums the
+// per-block partials (fixed order, the same in every block) into LDS:
+// c[i] = sum_b partial[b][i]; block 0 also writes c (+ add) to hout.  Then
+// w[t] -= sum_i V[i][t] c[i] over the block's points (grid-stride).
+__global__ __launch_bounds__(KB) void reduce_combine_kernel(int m, int64_t n, int64_t ld,
+                                                            const double *__restrict__ V,
+                                                            const double *__restrict__ partial,
+                                                            int nb,
+                                                            const double *__restrict__ add,
+                                                            double *__restrict__ hout,
+                                                            double *__restrict__ w) {
+    extern __shared__ double c[];
+    for (int i = threadIdx.x; i < m; i += KB) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += partial[(size_t)b * m + i];
+        c[i] = s;
+        if (blockIdx.x == 0 && hout) hout[i] = add ? s + add[i] : s;
+    }
+    __syncthreads();
+    for (int64_t t = (int64_t)blockIdx.x * KB + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * KB) {
+        double s = 0.0;
+        for (int i = 0; i < m; ++i) s += V[(size_t)i * ld + t] * c[i];
+        w[t] = w[t] - s;
+    }
+}
+
+// |w| from the norm partials (every block, same order), written to *hnorm
+// by block 0, and vnext (vnext2) = w / |w| when given.
+__global__ __launch_bounds__(KB) void norm_normalize_kernel(int64_t n,
+                                                            const double *__restrict__ w,
+                                                            const double *__restrict__ partial,
+                                                            int nb, double *__restrict__ hnorm,
+                                                            double *__restrict__ vnext,
+                                                            double *__restrict__ vnext2) {
+    __shared__ double nrm;
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += partial[b];
+        s = sqrt(s);
+        nrm = s;
+        if (blockIdx.x == 0) *hnorm = s;
+    }
+    __syncthreads();
+    if (!vnext) return;
+    const double d = nrm;
+    for (int64_t t = (int64_t)blockIdx.x * KB + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * KB) {
+        const double v = w[t] / d;
+        vnext[t] = v;
+        if (vnext2) vnext2[t] = v;
+    }
 }
 
 inline int n_blocks(int64_t n) {
@@ -115,26 +140,23 @@ extern "C" int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w
     hipStream_t st = as_stream(stream);
     double *partial = scratch;
     double *h1 = scratch + (size_t)nb * (m + 1);
-    const unsigned gp = (unsigned)ceil_div(n, KB);
-    const unsigned gr = (unsigned)ceil_div(m + 1, 64);
+    // update launches: enough blocks to cover the points, at most 1024
+    const int64_t gpl = ceil_div(n, KB);
+    const unsigned gp = (unsigned)(gpl < 1024 ? gpl : 1024);
+    const size_t lds = sizeof(double) * (size_t)m;
+    TCT_CHECK_ARG(lds <= 60 * 1024, "cgs2: basis size %d too large for LDS", m);
     // pass 1: h1 = V w; w -= V^T h1
     hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, m, n, ld, V, w, 0, partial);
-    hipLaunchKernelGGL(reduce_cols_kernel, dim3(gr), dim3(64), 0, st, m, nb, partial,
-                       (const double *)nullptr, 0, h1);
-    hipLaunchKernelGGL(combine_kernel, dim3(gp), dim3(KB), 0, st, m, n, ld, V, h1, -1.0, 1.0, w);
-    // pass 2: h2 = V w; w -= V^T h2; h = h1 + h2
+    hipLaunchKernelGGL(reduce_combine_kernel, dim3(gp), dim3(KB), lds, st, m, n, ld, V, partial,
+                       nb, (const double *)nullptr, h1, w);
+    // pass 2: h2 = V w; w -= V^T h2; h = h2 + h1
     hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, m, n, ld, V, w, 0, partial);
-    hipLaunchKernelGGL(reduce_cols_kernel, dim3(gr), dim3(64), 0, st, m, nb, partial,
-                       (const double *)nullptr, 0, h);
-    hipLaunchKernelGGL(combine_kernel, dim3(gp), dim3(KB), 0, st, m, n, ld, V, h, -1.0, 1.0, w);
-    hipLaunchKernelGGL(reduce_cols_kernel, dim3(gr), dim3(64), 0, st, m, 1, h, h1, 0, h);
-    // h[m] = |w|
+    hipLaunchKernelGGL(reduce_combine_kernel, dim3(gp), dim3(KB), lds, st, m, n, ld, V, partial,
+                       nb, (const double *)h1, h, w);
+    // h[m] = |w|, and the next basis vector
     hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, 0, n, ld, V, w, 1, partial);
-    hipLaunchKernelGGL(reduce_cols_kernel, dim3(1), dim3(64), 0, st, 1, nb, partial,
-                       (const double *)nullptr, 1, h + m);
-    if (vnext)
-        hipLaunchKernelGGL(normalize_kernel, dim3(gp), dim3(KB), 0, st, n, w, h + m, vnext,
-                           vnext2);
+    hipLaunchKernelGGL(norm_normalize_kernel, dim3(vnext ? gp : 1), dim3(KB), 0, st, n, w,
+                       partial, nb, h + m, vnext, vnext2);
     TCT_CHECK_LAUNCH("cgs2");
     return TCT_OK;
 }

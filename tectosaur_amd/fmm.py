@@ -224,10 +224,10 @@ class TreeFarfield:
         self.seg_off = torch.as_tensor(seg_off, device=dev)
         self.chunks = 1  # the reduced output has one chunk
         # upward pass: one launch per level, only up to the highest level any
-        # list refers to (one launch for all levels, where the block that
-        # finishes a parent's last child goes on to the parent, measured
-        # 1.4x slower at 32k triangles: its device-scope fences between
-        # levels cost more than the launches they save)
+        # list refers to.  Two one-launch forms measured slower at 4k-32k
+        # triangles: the block that finishes a parent's last child going on
+        # to the parent (device-scope fences between levels, 1.4x), and one
+        # workgroup forming the small top levels in turn (3x at 4k)
         used = lst[lst >= 0]
         self.max_level = int(self.tree.level[used].max()) if used.size else 0
         self._moments = {}

@@ -76,3 +76,4 @@ def test_treecode_all_direct_equals_direct(K, ft, tol):
     assert op.tree.stats['m2p'] == 0
     assert np.linalg.norm(y2 - y1) / np.linalg.norm(y1) < tol
 
+
