@@ -334,6 +334,11 @@ int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w, double *h
 int tct_krylov_combine(int m, int64_t n, int64_t ld, const double *V, const double *c,
                        double alpha, double beta, double *out, void *stream);
 
+/* tct_set_host_threads: OpenMP threads for the host utilities below when
+ * called from the calling thread (no reference counterpart; the reference's
+ * OpenMP uses its default). */
+int tct_set_host_threads(int n);
+
 /* ---------------- host-side mesh utilities (CPU, OpenMP) -----------------
  * Reference: fast_find_nearfield.get_nearfield + split_adjacent_close
  * (tectosaur/mesh/fast_find_nearfield.cpp:22-217).

@@ -84,6 +84,7 @@ SIGNATURES = {
     'tct_bsr9_spmv_f32': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_csr_spmv': [_i64, _vp, _vp, _vp, _vp, _vp, _int, _vp],
     'tct_find_near_pairs': [_i64, _vp, _vp, _i64, _vp, _vp, _dbl, _vp, _i64, _vp],
+    'tct_set_host_threads': [_int],
     'tct_split_adjacent_close': [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'tct_build_constraint_matrix': [_i64, _vp, _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
                                     _vp, _vp, _vp, _vp, _vp],

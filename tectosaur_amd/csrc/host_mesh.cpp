@@ -9,6 +9,8 @@
 // B centroids (cell >= the largest possible reach), evaluates the same
 // predicate with the same floating-point operation order (this file is
 // compiled with -ffp-contract=off), and returns the pairs sorted by (i, j).
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -87,6 +89,15 @@ void build_grid(Grid &g, int64_t n, const double *c, double cell) {
 }
 
 }  // namespace
+
+// Threads of the OpenMP regions that the host utilities start from the
+// calling thread (a server's preparation workers keep a few cores free for
+// the thread that drives the device).
+extern "C" int tct_set_host_threads(int n) {
+    if (n < 1) return tct::set_error(TCT_EARG, "set_host_threads: n %d", n);
+    omp_set_num_threads(n);
+    return TCT_OK;
+}
 
 extern "C" int tct_find_near_pairs(int64_t n_a, const double *ca, const double *ra, int64_t n_b,
                                    const double *cb, const double *rb, double threshold,

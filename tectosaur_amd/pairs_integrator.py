@@ -10,6 +10,8 @@ summation); results are returned as float64.
 The ``*_dev`` variants take and return device tensors, for assembling the
 nearfield matrix without host round trips.
 """
+import threading
+
 import numpy as np
 import torch
 
@@ -19,6 +21,10 @@ from .quadrature import gauss4d_tri
 
 # pairs per launch: bounds the output buffer of one call (81 doubles/pair)
 CALL_SIZE = 1 << 20
+
+
+_RULES = {}
+_RULES_LOCK = threading.Lock()
 
 
 class PairsIntegrator:
@@ -35,9 +41,9 @@ class PairsIntegrator:
         self.gpu_pts = torch.as_tensor(np.ascontiguousarray(pts, dtype=np.float64), **f64)
         self.gpu_tris = torch.as_tensor(np.ascontiguousarray(tris, dtype=np.int32),
                                         device=self.device)
-        self.near_q = self.quad_to_gpu(gauss4d_tri(nq_near, nq_near))
-        self.far_q = self.quad_to_gpu(gauss4d_tri(nq_far, nq_far))
         self._rules = {}
+        self.near_q = self._rule(('near4d', nq_near), lambda: gauss4d_tri(nq_near, nq_near))
+        self.far_q = self._rule(('far4d', nq_far), lambda: gauss4d_tri(nq_far, nq_far))
 
     def quad_to_gpu(self, q):
         f64 = dict(dtype=torch.float64, device=self.device)
@@ -46,7 +52,16 @@ class PairsIntegrator:
 
     def _rule(self, key, make):
         if key not in self._rules:
-            self._rules[key] = self.quad_to_gpu(make())
+            # rules are pure functions of their orders and read-only on the
+            # device: one copy per device shared by every integrator
+            dkey = (key, str(self.device))
+            q = _RULES.get(dkey)
+            if q is None:
+                with _RULES_LOCK:
+                    q = _RULES.get(dkey)
+                    if q is None:
+                        q = _RULES[dkey] = self.quad_to_gpu(make())
+            self._rules[key] = q
         return self._rules[key]
 
     def pairs_quad_dev(self, q, pairs_dev, check0, out=None, lanes_per_pair=0):

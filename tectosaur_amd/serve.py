@@ -29,6 +29,7 @@ default) the batched path is compared with.
 import collections
 import concurrent.futures
 import sys
+import threading
 import time
 
 import numpy as np
@@ -102,7 +103,17 @@ def _release(a):
     a.prob = None
 
 
-def _prepare(req, orders, tree=None):
+_PREP_LOCAL = threading.local()
+# OpenMP threads per preparation worker (four workers x 4 on the 16-core
+# share of one GPU; all 16 per worker measured no faster)
+PREP_OMP_THREADS = 4
+
+
+def _prepare(req, orders, tree=None, omp_threads=None):
+    if omp_threads and getattr(_PREP_LOCAL, 'omp', None) != omp_threads:
+        from . import _native as N
+        N.call('tct_set_host_threads', omp_threads)
+        _PREP_LOCAL.omp = omp_threads
     t0 = time.perf_counter()
     pts, tris = fault_mesh(req)
     t = request_traction(req, tris.shape[0])
@@ -116,6 +127,10 @@ def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None)
     prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
     b = op.cmT.dot_dev(prob.rhs)
+    if mode == 'tree' and prob.H.farfield.tree is None:
+        # below the treecode's size threshold the inner operator is the FP32
+        # direct farfield: refine as the mixed mode does (deeper cycles)
+        mode = 'mixed'
     state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
     if graphs:
         op.enable_graphs(MODES[mode][0:2] + MODES[mode][3:4], stream=capture_stream)
@@ -169,6 +184,7 @@ class Server:
         self.capture_stream = torch.cuda.Stream() if graphs else None
         self._next_stream = 0
         self.ticks = 0
+        self.max_concurrent = 0   # most solves active in one tick
 
     def _build_on_stream(self, prep):
         """Device build of one solve on the build stream (builder thread);
@@ -267,7 +283,7 @@ class Server:
                     if realtime and pending[si].arrival > now:
                         break
                     futures[pending[si].rid] = pool.submit(_prepare, pending[si], self.orders,
-                                                           self.tree)
+                                                           self.tree, PREP_OMP_THREADS)
                     si += 1
                 # admission (FIFO) of prepared requests
                 while (qi < len(pending) and
@@ -335,6 +351,7 @@ class Server:
                     else:
                         inflight.append(self._issue(ready))
                     continue
+                self.max_concurrent = max(self.max_concurrent, len(active))
                 members, sizes, pinned, ev = inflight.popleft()
                 ev.synchronize()
                 host = pinned.numpy()

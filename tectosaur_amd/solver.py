@@ -72,7 +72,10 @@ class ConstrainedOperator:
         for prec in precs:
             if prec is None or prec in self._graphs:
                 continue
-            self._apply_raw(self._gx, self._gy, prec)      # allocate lazy buffers eagerly
+            if hasattr(self.A, 'prepare'):
+                self.A.prepare(prec)                       # lazy buffers, no work
+            else:
+                self._apply_raw(self._gx, self._gy, prec)  # allocate them by running once
             side.wait_stream(cur)
             g = torch.cuda.CUDAGraph()
             with torch.cuda.stream(side):

@@ -177,6 +177,15 @@ class TriToTriDirectFarfieldOp:
         self._scatter(prec, out_dev)
         return out_dev
 
+    def prepare(self, prec):
+        """Allocate the buffers a ``dot_dev(prec)`` uses, without running it
+        (so a graph capture allocates nothing of its own)."""
+        if prec == 'tree' and self.tree is not None:
+            self.tree.moments('f32')
+            self.tree.partial
+        else:
+            self.engine.partial('f32' if prec == 'tree' else prec)
+
     def dot(self, v):
         v_dev = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device=self.device)
         return self.dot_dev(v_dev).cpu().numpy()

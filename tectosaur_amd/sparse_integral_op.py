@@ -50,6 +50,11 @@ class RegularizedSparseIntegralOp:
         self.nearfield.dot_dev(v_dev, out, accumulate=True, prec=prec or 'f64')
         return out
 
+    def prepare(self, prec):
+        """Allocate the lazily created buffers of ``dot_dev(prec)``."""
+        if hasattr(self.farfield, 'prepare'):
+            self.farfield.prepare(prec)
+
     def dot(self, v):
         x = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device=self.device)
         return self.dot_dev(x).cpu().numpy()
