@@ -38,8 +38,8 @@ RH_NEAR_FLOPS_PER_QP = 65
 # HBM bytes per launch from hardware counters (rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE in separate runs, scaled x2 and x1 by a 1 GiB copy measured in
 # the same runs): profiles/far_p2p_H_two_surface_442k_pmc_traffic.md.
-TRAFFIC_FAR_P2P_H_F32 = 2.389e9
-TRAFFIC_BSR_ICOSPHERE_L5 = 1.345e9
+TRAFFIC_FAR_P2P_H_F32 = 1.881e9
+TRAFFIC_BSR_ICOSPHERE_L5 = 1.330e9
 TRAFFIC_SOURCE = 'profiles/far_p2p_H_two_surface_442k_pmc_traffic.md'
 
 
@@ -365,27 +365,30 @@ def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
         {'f64_vs_oracle_rel_l2': rl2(out64_rows), 'f32_vs_oracle_rel_l2': rl2(out32_rows)}
 
 
-def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512):
+def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
+                modes=('tree', 'f64')):
     """Served traction -> slip solves: one Poisson stream of fault requests
     through the continuous-batching server (FP32 treecode farfield in the
     Arnoldi cycles, and FP32 direct) with FP64 refinement, and through the
     plain FP64 one-at-a-time path; latency = finish - arrival (queueing,
     device setup and solve).  Every result's residual is the FP64 direct
-    operator's."""
+    operator's.  ``modes``: 'tree' and 'mixed' (batched), 'f64' (sequential,
+    the baseline; always run)."""
     import torch
     from tectosaur_amd.serve import Server, summarize
     from tectosaur_amd.traction_slip import random_requests
     warm = random_requests(2, rate=0.0, seed=seed + 99, n_min=512, n_max=1024)
-    Server(mode='tree', max_active=2).run(warm, realtime=False)
-    Server(mode='mixed', max_active=2).run(warm, realtime=False)
+    for m in modes:
+        if m in ('tree', 'mixed'):
+            Server(mode=m, max_active=2).run(warm, realtime=False)
     Server(mode='f64', max_active=1, prep_workers=0, graphs=False).run(warm, realtime=False)
     reqs = random_requests(n_req, rate=rate, seed=seed, n_min=min_tris, n_max=max_tris)
     out = {'workload': '%d fault requests, Poisson %.2f/s, strike U(0,360) dip U(20,90) '
                        'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near and'
                        ' far, tol 1e-8' % (n_req, rate, min_tris, max_tris)}
-    for name, mode, ma, pw in (('batched_tree', 'tree', max_active, 4),
-                               ('batched_mixed', 'mixed', max_active, 4),
-                               ('sequential_f64', 'f64', 1, 0)):
+    runs = [('batched_' + m, m, max_active, 4) for m in ('tree', 'mixed') if m in modes]
+    runs.append(('sequential_f64', 'f64', 1, 0))
+    for name, mode, ma, pw in runs:
         srv = Server(mode=mode, max_active=ma, prep_workers=pw, graphs=(ma > 1))
         torch.cuda.synchronize()
         recs = srv.run(reqs, realtime=True)
@@ -394,12 +397,13 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512):
         out[name] = sm
         print('# serve %s: %s' % (name, json.dumps(sm)), flush=True)
     f = out['sequential_f64']
-    for name in ('batched_tree', 'batched_mixed'):
+    for name, _, _, _ in runs[:-1]:
         b = out[name]
         out[name]['speedup_solves_per_s'] = b['solves_per_s'] / f['solves_per_s']
         out[name]['p50_ratio'] = f['p50_s'] / b['p50_s']
         out[name]['p99_ratio'] = f['p99_s'] / b['p99_s']
-    out['speedup_solves_per_s'] = out['batched_tree']['speedup_solves_per_s']
+    if 'batched_tree' in out:
+        out['speedup_solves_per_s'] = out['batched_tree']['speedup_solves_per_s']
     return out
 
 
@@ -411,7 +415,10 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-rows', type=int, default=4096)
     ap.add_argument('--kernels', default='elasticU3,elasticT3,elasticA3,elasticH3')
-    ap.add_argument('--serve-requests', type=int, default=48)
+    ap.add_argument('--serve-requests', type=int, default=256)
+    ap.add_argument('--serve-modes', default='tree',
+                    help="batched modes to run beside the sequential FP64 path: 'tree', "
+                         "'mixed' or 'tree,mixed'")
     ap.add_argument('--serve-max-tris', type=int, default=32768)
     ap.add_argument('--serve-rate', type=float, default=100.0)
     ap.add_argument('--no-serve', action='store_true')
@@ -515,7 +522,8 @@ def main():
     if not args.no_pairfind:
         line['pair_finding'] = pairfind_bench(pts, tris)
     if not args.no_serve:
-        line['served'] = serve_bench(args.serve_requests, args.serve_max_tris, args.serve_rate)
+        line['served'] = serve_bench(args.serve_requests, args.serve_max_tris, args.serve_rate,
+                                     modes=tuple(args.serve_modes.split(',')) + ('f64',))
     if not args.no_cpu_baseline:
         rng = np.random.default_rng(1)
         rows = np.sort(rng.choice(n, size=min(args.cpu_rows, n), replace=False))

@@ -5,8 +5,8 @@ admits an arrived request as soon as a slot is free and its estimated
 device footprint fits the free-HBM budget, builds its operators on the
 device, and from then on keeps its whole Krylov state in HBM.  Every tick
 advances a batch of active solves by one GMRES step: each solve enqueues
-its device work (matvec as a HIP-graph replay, orthogonalisation) on the
-stream, the small per-solve results (Hessenberg column or residual norm)
+its device work (matvec as a HIP-graph replay, orthogonalisation) on its
+solve stream, the small per-solve results (Hessenberg column or residual norm)
 are concatenated on the device and cross to the host in one asynchronous
 transfer, and the host-side Givens updates run.  The active solves are
 split into two batches whose ticks alternate, so the device works on one
@@ -17,7 +17,9 @@ arriving meanwhile join the next batch issued.
 Host-side set-up of arriving requests (mesh, pair classification, farfield
 layout, mass and constraint matrices) runs on a pool of worker threads while
 the device iterates the active solves; admission then only does the device
-half (pair integration, packing, uploads).
+half (pair integration, packing, uploads), on a builder thread and a build
+stream of its own.  Active solves are spread over a few solve streams, so
+small solves' kernels overlap the large ones'.
 
 Each solve's arithmetic does not depend on which other solves share its
 ticks (its kernels read only its own buffers), so a result is the same alone
