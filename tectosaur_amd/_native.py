@@ -141,6 +141,31 @@ def require_device():
     return torch.device('cuda', torch.cuda.current_device())
 
 
+def upload(a, dtype=None, device=None):
+    """Host array -> new device tensor without waiting for the device.
+
+    The data is copied into pinned host memory (a new pinned tensor from
+    torch's caching host allocator) and from there to the device
+    asynchronously on the current stream; the pinned block is reused only
+    after that copy has completed.  A pageable ``torch.as_tensor(a,
+    device=...)`` instead synchronises the stream and spins the calling
+    thread until all work queued before it is done.  Device tensors are
+    returned as they are (converted to ``dtype`` if given)."""
+    import numpy as np
+    if isinstance(a, torch.Tensor) and a.device.type != 'cpu':
+        return a if dtype is None or a.dtype == dtype else a.to(dtype)
+    t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    if device is None:
+        device = torch.device('cuda', torch.cuda.current_device())
+    if t.numel() == 0:
+        return torch.empty(t.shape, dtype=t.dtype, device=device)
+    p = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    p.copy_(t)
+    return p.to(device, non_blocking=True)
+
+
 def params_array(params):
     import numpy as np
     return np.ascontiguousarray(np.array(params, dtype=np.float64))

@@ -32,10 +32,28 @@ def _size_class(n):
     return int(math.ceil(math.log2(max(n, 1))))
 
 
+_ARCH = {}
+
+
+def device_arch(device):
+    """gfx architecture name of a device, looked up once per device under a
+    lock (torch's device-property query is not safe to run for the first
+    time from several threads at once)."""
+    idx = torch.device(device).index
+    if idx is None:
+        idx = torch.cuda.current_device()
+    with _LOCK:
+        arch = _ARCH.get(idx)
+        if arch is None:
+            arch = torch.cuda.get_device_properties(idx).gcnArchName.split(':')[0]
+            _ARCH[idx] = arch
+    return arch
+
+
 def _key(K_name, prec, n_obs_pad, n_src_pad, device):
     arch = 'gfx950'
     if device is not None and torch.cuda.is_available():
-        arch = torch.cuda.get_device_properties(device).gcnArchName.split(':')[0]
+        arch = device_arch(device)
     return '%s|%s|%s|o%d|s%d' % (arch, K_name, prec, _size_class(n_obs_pad),
                                  _size_class(n_src_pad))
 

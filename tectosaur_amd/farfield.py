@@ -107,10 +107,10 @@ class FarfieldEngine:
         dev = self.device
         f64 = dict(dtype=torch.float64, device=dev)
         st = N.stream_ptr()
-        op = torch.as_tensor(np.ascontiguousarray(obs_pts, dtype=np.float64), **f64)
-        on = torch.as_tensor(np.ascontiguousarray(obs_ns, dtype=np.float64), **f64)
-        sp = torch.as_tensor(np.ascontiguousarray(src_pts, dtype=np.float64), **f64)
-        sn = torch.as_tensor(np.ascontiguousarray(src_ns, dtype=np.float64), **f64)
+        op = N.upload(obs_pts, torch.float64, dev)
+        on = N.upload(obs_ns, torch.float64, dev)
+        sp = N.upload(src_pts, torch.float64, dev)
+        sn = N.upload(src_ns, torch.float64, dev)
         self.loc32 = torch.empty(6 * self.n_obs_pad, dtype=torch.float32, device=dev)
         self.obs_org = torch.empty(3 * (self.n_obs_pad // GROUP), **f64)
         self.obs_soa64 = torch.empty(6 * self.n_obs_pad, **f64)

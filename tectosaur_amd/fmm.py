@@ -209,8 +209,8 @@ class TreeFarfield:
             raise ValueError('treecode plan does not match the engine layout')
         dev = engine.device
         self.n_obs_tiles = engine.n_obs_pad // GROUP
-        self.list = torch.as_tensor(lst, device=dev)
-        self.node_cr = torch.as_tensor(self.tree.node_cr, dtype=torch.float64, device=dev)
+        self.list = N.upload(lst, None, dev)
+        self.node_cr = N.upload(self.tree.node_cr, torch.float64, dev)
         self.C = N_CHANNELS[engine.K.name]
         self.T = n_terms(order)
         if n_waves is None:
@@ -220,8 +220,8 @@ class TreeFarfield:
             n_waves = 64 * 256
         seg, seg_off, self.seg_len = segments(start, n_waves)
         self.n_seg = seg.shape[0]
-        self.seg = torch.as_tensor(seg, device=dev)
-        self.seg_off = torch.as_tensor(seg_off, device=dev)
+        self.seg = N.upload(seg, None, dev)
+        self.seg_off = N.upload(seg_off, None, dev)
         self.chunks = 1  # the reduced output has one chunk
         # upward pass: one launch per level, only up to the highest level any
         # list refers to.  Two one-launch forms measured slower at 4k-32k

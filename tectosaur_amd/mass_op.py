@@ -20,9 +20,9 @@ class DeviceCSR:
         dev = N.require_device()
         self.device = dev
         self.shape = A.shape
-        self.indptr = torch.as_tensor(A.indptr.astype(np.int32), device=dev)
-        self.indices = torch.as_tensor(A.indices.astype(np.int32), device=dev)
-        self.data = torch.as_tensor(A.data.astype(np.float64), device=dev)
+        self.indptr = N.upload(A.indptr.astype(np.int32), None, dev)
+        self.indices = N.upload(A.indices.astype(np.int32), None, dev)
+        self.data = N.upload(A.data.astype(np.float64), None, dev)
 
     def dot_dev(self, x, out=None, accumulate=False):
         if out is None:

@@ -148,14 +148,14 @@ class RegularizedNearfieldIntegralOp:
                 continue
             from .pairs_integrator import check_pairs
             check_pairs(prs, tris.shape[0])
-            p_dev = torch.as_tensor(np.ascontiguousarray(prs.astype(np.int32)), device=dev)
+            p_dev = N.upload(prs.astype(np.int32), None, dev)
             acc = pairs_int.pairs_quad_dev(rules[name], p_dev, check0)
             corr = corr_int.pairs_quad_dev(far_q, p_dev[:, :2].contiguous(), corr_check0)
             diff = torch.empty_like(acc)
             N.call('tct_sub_f64', acc.numel(), N.ptr(acc), N.ptr(corr), N.ptr(diff),
                    N.stream_ptr())
-            rows.append(torch.as_tensor(dofs[:, 0], device=dev))
-            cols.append(torch.as_tensor(dofs[:, 1], device=dev))
+            rows.append(N.upload(dofs[:, 0], None, dev))
+            cols.append(N.upload(dofs[:, 1], None, dev))
             corrected.append(diff)
             if keep_uncorrected:
                 uncorrected.append(acc)

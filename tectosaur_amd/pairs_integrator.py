@@ -37,18 +37,15 @@ class PairsIntegrator:
         tris = np.asarray(tris)
         if tris.size and (tris.min() < 0 or tris.max() >= np.asarray(pts).shape[0]):
             raise IndexError('triangle vertex index out of range')
-        f64 = dict(dtype=torch.float64, device=self.device)
-        self.gpu_pts = torch.as_tensor(np.ascontiguousarray(pts, dtype=np.float64), **f64)
-        self.gpu_tris = torch.as_tensor(np.ascontiguousarray(tris, dtype=np.int32),
-                                        device=self.device)
+        self.gpu_pts = N.upload(np.asarray(pts, dtype=np.float64), None, self.device)
+        self.gpu_tris = N.upload(np.asarray(tris, dtype=np.int32), None, self.device)
         self._rules = {}
         self.near_q = self._rule(('near4d', nq_near), lambda: gauss4d_tri(nq_near, nq_near))
         self.far_q = self._rule(('far4d', nq_far), lambda: gauss4d_tri(nq_far, nq_far))
 
     def quad_to_gpu(self, q):
-        f64 = dict(dtype=torch.float64, device=self.device)
-        return (torch.as_tensor(np.ascontiguousarray(q[0]), **f64),
-                torch.as_tensor(np.ascontiguousarray(q[1]), **f64))
+        return (N.upload(q[0], torch.float64, self.device),
+                N.upload(q[1], torch.float64, self.device))
 
     def _rule(self, key, make):
         if key not in self._rules:
@@ -85,8 +82,7 @@ class PairsIntegrator:
         if n == 0:
             return np.empty((0, 3, 3, 3, 3), dtype=np.float64)
         check_pairs(pairs_list, self.gpu_tris.shape[0])
-        p = torch.as_tensor(np.ascontiguousarray(pairs_list.astype(np.int32)),
-                            device=self.device)
+        p = N.upload(pairs_list.astype(np.int32), None, self.device)
         return self.pairs_quad_dev(q, p, check0).cpu().numpy().reshape((n, 3, 3, 3, 3))
 
     def correction(self, pairs_list, check0):

@@ -30,6 +30,7 @@ default) the batched path is compared with.
 """
 import collections
 import concurrent.futures
+import contextlib
 import sys
 import threading
 import time
@@ -87,22 +88,30 @@ def estimate_bytes(n_tris, restart, tree=False):
 
 
 class _Active:
-    def __init__(self, rec, prob, op, state, nbytes, stream=None):
+    def __init__(self, rec, prob, op, state, nbytes, stream=None, build_stream=None):
         self.rec = rec
         self.prob = prob
         self.op = op
         self.state = state
         self.nbytes = nbytes
         self.stream = stream  # the stream all of this solve's steps run on
+        self.build_stream = build_stream  # the stream its buffers were allocated on
         self.busy = False     # a step is in flight
 
 
-def _release(a):
+def _release(a, capture_lock=None):
     """Drop a finished solve's device objects (and with them its HIP graphs)
-    on the calling thread."""
-    a.op = None
-    a.state = None
-    a.prob = None
+    on the calling thread, never during a graph capture (``capture_lock``).
+    Its buffers were allocated on its build stream and return to that
+    stream's pool: the build stream first waits for whatever is still queued
+    on the solve's stream, so a later build cannot reuse the memory under
+    it."""
+    if a.build_stream is not None and a.stream is not None:
+        a.build_stream.wait_stream(a.stream)
+    with capture_lock or contextlib.nullcontext():
+        a.op = None
+        a.state = None
+        a.prob = None
 
 
 _PREP_LOCAL = threading.local()
@@ -124,7 +133,8 @@ def _prepare(req, orders, tree=None, omp_threads=None):
     return prep
 
 
-def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None):
+def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None,
+           capture_lock=None):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
     prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
@@ -135,7 +145,12 @@ def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None)
         mode = 'mixed'
     state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
     if graphs:
-        op.enable_graphs(MODES[mode][0:2] + MODES[mode][3:4], stream=capture_stream)
+        precs = MODES[mode][0:2] + MODES[mode][3:4]
+        if capture_lock is None:
+            op.enable_graphs(precs, stream=capture_stream)
+        else:
+            with capture_lock:
+                op.enable_graphs(precs, stream=capture_stream)
     return prob, op, state
 
 
@@ -154,7 +169,12 @@ def _finalize(a, now, keep_slip):
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
                  hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None,
-                 graphs=True, streams=4, switch_interval=1e-3):
+                 graphs=True, streams=4, switch_interval=1e-3, builders=1):
+        # device queries from worker threads start from cached values
+        from . import _native as N
+        from .autotune import device_arch
+        device_arch(N.require_device())
+        torch.cuda.device_count()
         self.mode = mode
         self.max_active = max_active
         self.tol = tol
@@ -175,30 +195,49 @@ class Server:
         self.graphs = graphs
         # batched servers build new solves on a stream of their own
         # (default priority) so admissions overlap the running solves
-        self.build_stream = torch.cuda.Stream() if max_active > 1 else None
+        # (``builders`` threads, one build stream each; builds upload through
+        # pinned memory and never wait for the device, so one thread keeps
+        # up with the 256-request stream, and more measured no faster)
+        n_b = builders if max_active > 1 else 0
+        self.build_streams = [torch.cuda.Stream() for _ in range(n_b)]
+        self._builder_local = threading.local()
+        self._builder_count = 0
+        self._builder_lock = threading.Lock()
         # batched servers spread the active solves over a few streams
         # (default priority): one solve's kernels stay in order on its
         # stream while small solves' kernels overlap the large ones'
         n_st = streams if max_active > 1 else 0
         self.streams = [torch.cuda.Stream() for _ in range(n_st)]
         # every graph capture of this server runs on this one stream (no
-        # further pooled streams are taken while solves run)
+        # further pooled streams are taken while solves run), one capture
+        # at a time
         self.capture_stream = torch.cuda.Stream() if graphs else None
+        self.capture_lock = threading.Lock()
         self._next_stream = 0
         self.ticks = 0
         self.max_concurrent = 0   # most solves active in one tick
+        self.sync_wait_s = 0.0    # loop time spent waiting for tick results
+
+    def _init_builder(self):
+        """Builder thread initialiser: the thread's own build stream."""
+        with self._builder_lock:
+            i = self._builder_count
+            self._builder_count += 1
+        self._builder_local.stream = self.build_streams[i % len(self.build_streams)]
 
     def _build_on_stream(self, prep):
-        """Device build of one solve on the build stream (builder thread);
-        returns its objects and an event the compute stream waits on."""
-        with torch.cuda.stream(self.build_stream):
+        """Device build of one solve on the calling builder thread's build
+        stream; returns its objects, an event the solve's stream waits on,
+        and the build stream."""
+        st = self._builder_local.stream
+        with torch.cuda.stream(st):
             prob, op, state = _build(prep, self.mode, self.tol, self.restart, self.maxiter,
-                                     self.graphs, self.capture_stream)
+                                     self.graphs, self.capture_stream, self.capture_lock)
             ev = torch.cuda.Event()
             ev.record()
-        return prob, op, state, ev
+        return prob, op, state, ev, st
 
-    def _adopt(self, active, rec, prob, op, state, need, clock, ev=None):
+    def _adopt(self, active, rec, prob, op, state, need, clock, ev=None, build_stream=None):
         rec.n_tris = prob.n_tris
         rec.pair_evals = prob.pair_evals
         rec.setup_s = clock() - rec.admitted
@@ -208,7 +247,7 @@ class Server:
             self._next_stream += 1
         if ev is not None:
             (stream or torch.cuda.current_stream()).wait_event(ev)
-        active.append(_Active(rec, prob, op, state, need, stream))
+        active.append(_Active(rec, prob, op, state, need, stream, build_stream))
 
     def _issue(self, batch):
         """Enqueue one step of every solve in ``batch``; their small results
@@ -275,8 +314,9 @@ class Server:
         self._pinned = [torch.empty(cap, dtype=torch.float64, pin_memory=True) for _ in range(2)]
         self._events = [torch.cuda.Event() for _ in range(2)]
         self._slot = 0
-        builder = (concurrent.futures.ThreadPoolExecutor(1) if self.build_stream is not None
-                   else None)
+        builder = (concurrent.futures.ThreadPoolExecutor(len(self.build_streams),
+                                                          initializer=self._init_builder)
+                   if self.build_streams else None)
         try:
             while qi < len(pending) or active or building:
                 now = clock()
@@ -333,8 +373,8 @@ class Server:
                 # adopt finished builds (block only when nothing else can run)
                 while building and (building[0][0].done() or not active):
                     fut, rec, need = building.popleft()
-                    prob, op, state, ev = fut.result()
-                    self._adopt(active, rec, prob, op, state, need, clock, ev)
+                    prob, op, state, ev, bst = fut.result()
+                    self._adopt(active, rec, prob, op, state, need, clock, ev, bst)
                 if not active:
                     if qi < len(pending):
                         time.sleep(max(0.0, min(0.002, pending[qi].arrival - clock())))
@@ -355,7 +395,9 @@ class Server:
                     continue
                 self.max_concurrent = max(self.max_concurrent, len(active))
                 members, sizes, pinned, ev = inflight.popleft()
+                tw = time.perf_counter()
                 ev.synchronize()
+                self.sync_wait_s += time.perf_counter() - tw
                 host = pinned.numpy()
                 self.ticks += 1
                 off = 0
@@ -376,9 +418,9 @@ class Server:
                         if log:
                             log(a.rec)
                         if builder is not None:
-                            # the solve's graphs are destroyed on the builder
-                            # thread, never while that thread is capturing
-                            builder.submit(_release, a)
+                            # the solve's graphs are destroyed on a builder
+                            # thread, never while a capture is under way
+                            builder.submit(_release, a, self.capture_lock)
                 if finished:
                     active = [a for a in active if id(a) not in finished]
         finally:

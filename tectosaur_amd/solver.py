@@ -8,7 +8,8 @@ the Krylov vectors, the operator and the orthogonalisation stay in HBM:
   * ``ConstrainedOperator`` applies cm^T (A (cm x)) with device CSR matvecs
     around the summed farfield + nearfield matvec;
   * ``GmresState`` is restarted GMRES(m) with classical Gram-Schmidt
-    repeated twice (CGS2, two GEMVs per pass) in FP64 on the device; only
+    repeated twice (CGS2) in FP64 on the device (the HIP kernels of
+    krylov.hip for device vectors); only
     the Hessenberg column (m + 2 doubles) crosses to the host per iteration
     for the Givens update.  It is a step machine (``issue`` enqueues device
     work and returns a small device tensor, ``consume`` takes its host
@@ -45,6 +46,7 @@ class ConstrainedOperator:
         self._y = torch.empty(cm.shape[0], dtype=torch.float64, device=self.device)
         self.n_matvec = {'f32': 0, 'f64': 0, 'tree': 0}
         self._graphs = {}
+        self._gx = self._gy = None
 
     def _apply_raw(self, x, out, prec):
         self.cm.dot_dev(x, self._u)
@@ -65,8 +67,12 @@ class ConstrainedOperator:
         fresh stream per solve would in time alias a stream other solves
         replay on."""
         n = self.shape[0]
-        self._gx = torch.zeros(n, dtype=torch.float64, device=self.device)
-        self._gy = torch.empty(n, dtype=torch.float64, device=self.device)
+        if self._gx is None:
+            # every graph of this operator reads and writes these two
+            # buffers: allocated once, so later captures keep earlier
+            # graphs valid
+            self._gx = torch.zeros(n, dtype=torch.float64, device=self.device)
+            self._gy = torch.empty(n, dtype=torch.float64, device=self.device)
         cur = torch.cuda.current_stream(self.device)
         side = stream if stream is not None else torch.cuda.Stream(self.device)
         for prec in precs:
@@ -207,6 +213,12 @@ class GmresState:
             N.call('tct_krylov_scratch', self.m + 1, n, ctypes.byref(nd))
             self._scratch = torch.empty(nd.value, dtype=torch.float64, device=dev)
             self._h = torch.empty(self.m + 2, dtype=torch.float64, device=dev)
+            # the cycle's least-squares solution goes up through pinned
+            # memory without a stream synchronisation; the pinned buffer is
+            # rewritten only at the next cycle's end, after later work on the
+            # same stream has been waited for
+            self._ypin = torch.empty(self.m, dtype=torch.float64, pin_memory=True)
+            self._ydev = torch.empty(self.m, dtype=torch.float64, device=dev)
         self.it = 0
         self.restarts = 0
         self.residuals = []
@@ -359,19 +371,23 @@ class GmresState:
                 R[:jj + 1, jj] = cj
             y = np.linalg.solve(R, np.asarray(self.g[:k]))
             if np.all(np.isfinite(y)):
-                y_t = torch.as_tensor(y, dtype=torch.float64, device=self.x.device)
                 if self._hip:
                     from . import _native as N
+                    self._ypin[:k].copy_(torch.from_numpy(y))
+                    y_t = self._ydev[:k]
+                    y_t.copy_(self._ypin[:k], non_blocking=True)
                     tgt, beta = (self.d, 0.0) if self.d is not None else (self.x, 1.0)
                     N.call('tct_krylov_combine', k, self.n, self.n, N.ptr(self.V), N.ptr(y_t),
                            1.0, beta, N.ptr(tgt), N.stream_ptr())
                     if self.d is not None:
                         self.x.add_(self.d)
-                elif self.d is not None:
-                    torch.mv(self.V[:k].t(), y_t, out=self.d)
-                    self.x.add_(self.d)
                 else:
-                    self.x.add_(torch.mv(self.V[:k].t(), y_t))
+                    y_t = torch.as_tensor(y, dtype=torch.float64, device=self.x.device)
+                    if self.d is not None:
+                        torch.mv(self.V[:k].t(), y_t, out=self.d)
+                        self.x.add_(self.d)
+                    else:
+                        self.x.add_(torch.mv(self.V[:k].t(), y_t))
             elif self.d is not None:
                 self.d.zero_()
         elif self.d is not None:

@@ -20,9 +20,9 @@ class BSRMatrix:
     def __init__(self, indptr, indices, data, shape):
         dev = N.require_device()
         self.device = dev
-        self.indptr = torch.as_tensor(indptr, device=dev).to(torch.int32).contiguous()
-        self.indices = torch.as_tensor(indices, device=dev).to(torch.int32).contiguous()
-        d = torch.as_tensor(data, device=dev).to(torch.float64)
+        self.indptr = N.upload(indptr, torch.int32, dev).contiguous()
+        self.indices = N.upload(indices, torch.int32, dev).contiguous()
+        d = N.upload(data, torch.float64, dev)
         self.data = d.reshape((-1, 81)).contiguous()
         self.data32 = None
         self.shape = tuple(shape)
@@ -100,11 +100,9 @@ class BCOOMatrix:
 
     def __init__(self, rows, cols, data, shape):
         dev = N.require_device()
-        self.rows = torch.as_tensor(np.asarray(rows) if not isinstance(rows, torch.Tensor)
-                                    else rows, device=dev).to(torch.int64)
-        self.cols = torch.as_tensor(np.asarray(cols) if not isinstance(cols, torch.Tensor)
-                                    else cols, device=dev).to(torch.int64)
-        self.data = torch.as_tensor(data, device=dev).to(torch.float64).reshape((-1, 81))
+        self.rows = N.upload(rows, torch.int64, dev)
+        self.cols = N.upload(cols, torch.int64, dev)
+        self.data = N.upload(data, torch.float64, dev).reshape((-1, 81))
         self.shape = tuple(shape)
         self._bsr = None
 

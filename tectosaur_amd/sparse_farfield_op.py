@@ -103,17 +103,15 @@ class TriToTriDirectFarfieldOp:
         self.engine = FarfieldEngine(K_name, params, plan['o_pts'], plan['o_ns'], plan['s_pts'],
                                      plan['s_ns'], precisions=precisions, device=dev,
                                      tile_cfg=tile_cfg, extents=plan['extents'])
-        i32 = dict(dtype=torch.int32, device=dev)
-        f64 = dict(dtype=torch.float64, device=dev)
         # source point k belongs to DOF-space triangle tri_of_pt[k]
-        self.tri_of_pt = torch.as_tensor(plan['tri_of_pt'], **i32)
-        self.pt_of_tri = torch.as_tensor(plan['pt_of_tri'], **i32)
-        self.qphi = torch.as_tensor(plan['phi'], **f64)
-        self.wj_src = torch.as_tensor(plan['s_wj'], **f64)
-        self.wj_obs = torch.as_tensor(plan['o_wj'], **f64)
+        self.tri_of_pt = N.upload(plan['tri_of_pt'], torch.int32, dev)
+        self.pt_of_tri = N.upload(plan['pt_of_tri'], torch.int32, dev)
+        self.qphi = N.upload(plan['phi'], torch.float64, dev)
+        self.wj_src = N.upload(plan['s_wj'], torch.float64, dev)
+        self.wj_obs = N.upload(plan['o_wj'], torch.float64, dev)
         if self.curl:
-            self.curl_src = torch.as_tensor(plan['curl_src'], **f64)
-            self.curl_obs = torch.as_tensor(plan['curl_obs'], **f64)
+            self.curl_src = N.upload(plan['curl_src'], torch.float64, dev)
+            self.curl_obs = N.upload(plan['curl_obs'], torch.float64, dev)
         self._plan_pts = (plan['o_pts'], plan['s_pts'])
         self.tree = None
 

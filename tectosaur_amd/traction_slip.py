@@ -30,6 +30,7 @@ import numpy as np
 import scipy.sparse
 import torch
 
+from . import _native as N
 from . import mesh as mesh_mod
 from .constraints import find_free_edges
 from .mass_op import MassOp
@@ -179,7 +180,7 @@ class TractionSlipProblem:
             self.H.farfield.attach_tree(order=tr['order'], mac=tr['mac'], plan=tr['plan'])
         self.M = prepared['M']
         self.cm = prepared['cm']
-        self.rhs = torch.as_tensor(prepared['rhs'], dtype=torch.float64, device=self.device)
+        self.rhs = N.upload(prepared['rhs'], torch.float64, self.device)
 
     @property
     def n_dofs(self):

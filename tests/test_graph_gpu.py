@@ -36,3 +36,28 @@ def test_captured_apply_equals_eager_on_every_replay():
                 torch.cuda.synchronize()
                 assert torch.equal(y, eager[p][i]), (rep, p, i)
                 assert torch.isfinite(z)
+
+
+def test_graphs_captured_in_separate_calls_stay_valid():
+    # enable_graphs called once per precision: the graph input and output
+    # buffers are shared, so the first graph still reads and writes them
+    from tectosaur_amd.solver import ConstrainedOperator
+    from tectosaur_amd.traction_slip import (FaultRequest, TractionSlipProblem, fault_mesh,
+                                             prepare_problem, request_traction)
+    req = FaultRequest(0, 120.0, 40.0, 6000.0, 2.0, 500, np.array([-0.4, 0.1, 0.6]), 2)
+    pts, tris = fault_mesh(req)
+    prep = prepare_problem(pts, tris, request_traction(req, tris.shape[0]), orders=ORD)
+    prob = TractionSlipProblem(None, None, None, prepared=prep)
+    op = ConstrainedOperator(prob.H, prob.cm)
+    x = torch.as_tensor(np.random.default_rng(1).normal(size=op.shape[0]), device='cuda')
+    eager = {p: op.apply(x, None, p).clone() for p in ('f32', 'f64')}
+    op.enable_graphs(('f32',))
+    gx = op.gx
+    op.enable_graphs(('f64',))
+    assert op.gx is gx
+    for p in ('f32', 'f64', 'f32'):
+        assert torch.equal(op.apply(x, None, p), eager[p]), p
+        op.gx.copy_(x)
+        op.replay(p)
+        torch.cuda.synchronize()
+        assert torch.equal(op.gy, eager[p]), p

@@ -107,7 +107,9 @@ def test_tree_server_result_same_alone_and_in_mix():
     from tectosaur_amd.traction_slip import random_requests
     reqs = random_requests(4, rate=0.0, seed=8, n_min=400, n_max=1500)
     tree = dict(order=4, mac=2.5, min_tris=0)
-    mix = Server(mode='tree', max_active=4, restart=100, orders=ORD, keep_slip=True, tree=tree)
+    # two builder threads (each with its own build stream; captures serialised)
+    mix = Server(mode='tree', max_active=4, restart=100, orders=ORD, keep_slip=True, tree=tree,
+                 builders=2)
     recs_mix = mix.run(reqs, realtime=False)
     for req, rm in zip(reqs, recs_mix):
         ra = Server(mode='tree', max_active=1, restart=100, orders=ORD, keep_slip=True,
