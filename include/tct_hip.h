@@ -334,6 +334,23 @@ int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w, double *h
 int tct_krylov_combine(int m, int64_t n, int64_t ld, const double *V, const double *c,
                        double alpha, double beta, double *out, void *stream);
 
+/* Serving-loop plumbing (no reference counterpart: the reference's solves
+ * are one at a time on the host).
+ * tct_arnoldi_step: one Arnoldi step of a solve whose operator application
+ *   is a captured graph: launch graph_exec (a hipGraphExec_t that reads the
+ *   graph input buffer and writes w) on stream, then tct_cgs2.
+ * tct_tick_gather: the results of one tick of many solves.  main_stream
+ *   first waits for each streams[i] (through events[i], re-recorded there),
+ *   then the n_seg device segments src[k][0, len[k]) are concatenated into
+ *   staging (staging_doubles long) and copied to host (pinned) memory, and
+ *   done_event (may be NULL) is recorded on main_stream. */
+int tct_arnoldi_step(const void *graph_exec, int m, int64_t n, int64_t ld, const double *V,
+                     double *w, double *h, double *scratch, int64_t scratch_doubles,
+                     double *vnext, double *vnext2, void *stream);
+int tct_tick_gather(int n_streams, void *const *streams, void *const *events, int n_seg,
+                    const double *const *src, const int *len, double *staging,
+                    int64_t staging_doubles, double *host, void *main_stream, void *done_event);
+
 /* tct_set_host_threads: OpenMP threads for the host utilities below when
  * called from the calling thread (no reference counterpart; the reference's
  * OpenMP uses its default). */

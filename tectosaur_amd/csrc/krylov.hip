@@ -113,6 +113,22 @@ __global__ __launch_bounds__(KB) void norm_normalize_kernel(int64_t n,
     }
 }
 
+// Up to GATHER_MAX device segments concatenated into one buffer: block k
+// copies segment k (kernel arguments by value, no table upload).
+constexpr int GATHER_MAX = 32;
+struct GatherSegs {
+    const double *src[GATHER_MAX];
+    int off[GATHER_MAX + 1];
+};
+
+__global__ __launch_bounds__(KB) void gather_segs_kernel(GatherSegs segs,
+                                                         double *__restrict__ out) {
+    const int k = blockIdx.x;
+    const double *src = segs.src[k];
+    const int o = segs.off[k], len = segs.off[k + 1] - o;
+    for (int i = threadIdx.x; i < len; i += KB) out[o + i] = src[i];
+}
+
 inline int n_blocks(int64_t n) {
     int64_t nb = (n + 4 * KB - 1) / (4 * KB);
     return (int)(nb < 1 ? 1 : (nb > 1024 ? 1024 : nb));
@@ -158,6 +174,63 @@ extern "C" int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w
     hipLaunchKernelGGL(norm_normalize_kernel, dim3(vnext ? gp : 1), dim3(KB), 0, st, n, w,
                        partial, nb, h + m, vnext, vnext2);
     TCT_CHECK_LAUNCH("cgs2");
+    return TCT_OK;
+}
+
+extern "C" int tct_arnoldi_step(const void *graph_exec, int m, int64_t n, int64_t ld,
+                                const double *V, double *w, double *h, double *scratch,
+                                int64_t scratch_doubles, double *vnext, double *vnext2,
+                                void *stream) {
+    TCT_CHECK_ARG(graph_exec, "arnoldi_step: null graph");
+    hipError_t e = hipGraphLaunch((hipGraphExec_t)graph_exec, as_stream(stream));
+    if (e != hipSuccess)
+        return set_error(TCT_EHIP, "arnoldi_step: graph launch: %s", hipGetErrorString(e));
+    return tct_cgs2(m, n, ld, V, w, h, scratch, scratch_doubles, vnext, vnext2, stream);
+}
+
+extern "C" int tct_tick_gather(int n_streams, void *const *streams, void *const *events,
+                               int n_seg, const double *const *src, const int *len,
+                               double *staging, int64_t staging_doubles, double *host,
+                               void *main_stream, void *done_event) {
+    TCT_CHECK_ARG(n_streams >= 0 && n_seg >= 0, "tick_gather: negative counts");
+    TCT_CHECK_ARG(n_streams == 0 || (streams && events), "tick_gather: null stream list");
+    TCT_CHECK_ARG(n_seg == 0 || (src && len && staging && host), "tick_gather: null pointer");
+    int64_t total = 0;
+    for (int k = 0; k < n_seg; ++k) {
+        TCT_CHECK_ARG(len[k] >= 0 && (len[k] == 0 || src[k]), "tick_gather: bad segment %d", k);
+        total += len[k];
+    }
+    TCT_CHECK_ARG(total <= staging_doubles, "tick_gather: %lld values exceed the staging buffer",
+                  (long long)total);
+    hipStream_t st = as_stream(main_stream);
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n_streams && e == hipSuccess; ++i) {
+        e = hipEventRecord((hipEvent_t)events[i], as_stream(streams[i]));
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, (hipEvent_t)events[i], 0);
+    }
+    if (e != hipSuccess)
+        return set_error(TCT_EHIP, "tick_gather: stream join: %s", hipGetErrorString(e));
+    int64_t off = 0;
+    for (int k0 = 0; k0 < n_seg; k0 += GATHER_MAX) {
+        const int nk = n_seg - k0 < GATHER_MAX ? n_seg - k0 : GATHER_MAX;
+        GatherSegs segs;
+        int64_t o = 0;
+        for (int k = 0; k < nk; ++k) {
+            segs.src[k] = src[k0 + k];
+            segs.off[k] = (int)o;
+            o += len[k0 + k];
+        }
+        segs.off[nk] = (int)o;
+        if (o > 0)
+            hipLaunchKernelGGL(gather_segs_kernel, dim3(nk), dim3(KB), 0, st, segs, staging + off);
+        off += o;
+    }
+    TCT_CHECK_LAUNCH("tick_gather");
+    if (total > 0)
+        e = hipMemcpyAsync(host, staging, sizeof(double) * (size_t)total, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && done_event) e = hipEventRecord((hipEvent_t)done_event, st);
+    if (e != hipSuccess)
+        return set_error(TCT_EHIP, "tick_gather: copy: %s", hipGetErrorString(e));
     return TCT_OK;
 }
 

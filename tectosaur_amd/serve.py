@@ -247,43 +247,47 @@ class Server:
             self._next_stream += 1
         if ev is not None:
             (stream or torch.cuda.current_stream()).wait_event(ev)
+        state.stream = stream
         active.append(_Active(rec, prob, op, state, need, stream, build_stream))
 
     def _issue(self, batch):
-        """Enqueue one step of every solve in ``batch``; their small results
-        are gathered on the device and copied to pinned host memory without
-        blocking.  Returns (batch, sizes, pinned buffer, event)."""
-        main = torch.cuda.current_stream()
-        if self.streams:
-            outs = []
-            used = {}
-            for a in batch:
-                with torch.cuda.stream(a.stream):
-                    outs.append(a.state.issue())
-                used[id(a.stream)] = a.stream
-            for st in used.values():
-                main.wait_stream(st)
-            for o in outs:
-                # read on this stream: keep the allocator from handing the
-                # block to the solve stream's next step before the gather
-                o.record_stream(main)
-        else:
-            outs = [a.state.issue() for a in batch]
+        """Enqueue one step of every solve in ``batch`` (each on its own
+        stream); then, in one native call, the main stream joins those
+        streams, gathers the small per-solve results into one device buffer
+        and copies it to pinned host memory without blocking.  Returns
+        (batch, sizes, pinned view, event, outputs); the outputs are kept
+        until consumed, after the event, so nothing they occupy is reused
+        while the gather may still read it."""
+        from . import _native as N
+        outs = []
+        used = {}
         for a in batch:
+            outs.append(a.state.issue())
             a.busy = True
-        dev = torch.cat(outs)
-        # one of two pinned buffers and events allocated up front: no host
-        # allocation (an unsafe call during another thread's graph capture)
-        # happens in the loop
+            if a.stream is not None:
+                used[id(a.stream)] = a.stream
         k = self._slot
         self._slot ^= 1
-        if self._pinned[k].shape[0] < dev.shape[0]:
+        src, ln = self._seg_src[k], self._seg_len[k]
+        n = len(outs)
+        if n > src.shape[0]:
+            raise RuntimeError('batch larger than the gather table')
+        for i, o in enumerate(outs):
+            src[i] = o.data_ptr()
+            ln[i] = o.numel()
+        total = int(ln[:n].sum())
+        if total > self._pinned[k].shape[0]:
             raise RuntimeError('batch results exceed the pinned buffer')
-        pinned = self._pinned[k][:dev.shape[0]]
-        pinned.copy_(dev, non_blocking=True)
+        sts = list(used.values())
+        for i, st in enumerate(sts):
+            self._join_st[i] = st.cuda_stream
+            self._join_ev[i] = self._join_events[id(st)].cuda_event
         ev = self._events[k]
-        ev.record()
-        return batch, [o.shape[0] for o in outs], pinned, ev
+        N.call('tct_tick_gather', len(sts), self._join_st.ctypes.data, self._join_ev.ctypes.data,
+               n, src.ctypes.data, ln.ctypes.data, self._staging[k].data_ptr(),
+               self._staging[k].shape[0], self._pinned[k].data_ptr(),
+               torch.cuda.current_stream().cuda_stream, ev.cuda_event)
+        return batch, ln[:n].tolist(), self._pinned[k][:total], ev, outs
 
     def run(self, requests, realtime=True, log=None):
         """Serve ``requests`` (sorted by arrival).  With realtime=True a
@@ -309,10 +313,25 @@ class Server:
         si = 0    # next request to hand to the preparation pool
         inflight = collections.deque()
         building = collections.deque()
-        # per batch at most max_active solves x (restart + 2) values
+        # per batch at most max_active solves x (restart + 2) values; two of
+        # everything (two batches in flight), allocated up front: no host
+        # allocation (an unsafe call during another thread's graph capture)
+        # happens in the loop
         cap = self.max_active * (self.restart + 2)
         self._pinned = [torch.empty(cap, dtype=torch.float64, pin_memory=True) for _ in range(2)]
+        self._staging = [torch.empty(cap, dtype=torch.float64, device='cuda') for _ in range(2)]
+        self._seg_src = [np.zeros(self.max_active, dtype=np.uint64) for _ in range(2)]
+        self._seg_len = [np.zeros(self.max_active, dtype=np.int32) for _ in range(2)]
         self._events = [torch.cuda.Event() for _ in range(2)]
+        # one join event per solve stream; each event is recorded once here
+        # so its handle exists before the native gather re-records it
+        self._join_events = {id(st): torch.cuda.Event() for st in self.streams}
+        self._join_st = np.zeros(max(1, len(self.streams)), dtype=np.uint64)
+        self._join_ev = np.zeros(max(1, len(self.streams)), dtype=np.uint64)
+        for st in self.streams:
+            self._join_events[id(st)].record(st)
+        for e in self._events:
+            e.record()
         self._slot = 0
         builder = (concurrent.futures.ThreadPoolExecutor(len(self.build_streams),
                                                           initializer=self._init_builder)
@@ -394,7 +413,7 @@ class Server:
                         inflight.append(self._issue(ready))
                     continue
                 self.max_concurrent = max(self.max_concurrent, len(active))
-                members, sizes, pinned, ev = inflight.popleft()
+                members, sizes, pinned, ev, _outs = inflight.popleft()
                 tw = time.perf_counter()
                 ev.synchronize()
                 self.sync_wait_s += time.perf_counter() - tw
@@ -404,11 +423,11 @@ class Server:
                 now = clock()
                 finished = set()
                 for a, sz in zip(members, sizes):
-                    # the host step may enqueue device work (basis vector,
-                    # solution update): on the solve's own stream
-                    with torch.cuda.stream(a.stream or torch.cuda.current_stream()):
-                        done = a.state.consume(host[off:off + sz])
-                        if done:
+                    # device work the host step enqueues (basis vector,
+                    # solution update) goes to the solve's own stream
+                    done = a.state.consume(host[off:off + sz])
+                    if done:
+                        with torch.cuda.stream(a.stream or torch.cuda.current_stream()):
                             _finalize(a, now, self.keep_slip)
                     off += sz
                     a.busy = False

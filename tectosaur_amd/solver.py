@@ -23,6 +23,7 @@ the Krylov vectors, the operator and the orthogonalisation stay in HBM:
 
 Tolerances are relative to ||rhs|| of the constrained system, as scipy's.
 """
+import contextlib
 import ctypes
 import math
 import time
@@ -97,6 +98,10 @@ class ConstrainedOperator:
 
     def has_graph(self, prec):
         return prec in self._graphs
+
+    def graph_exec(self, prec):
+        """The executable graph of ``prec`` (a hipGraphExec_t address)."""
+        return self._graphs[prec].raw_cuda_graph_exec()
 
     def replay(self, prec):
         """One captured application: graph_input -> graph_output (the
@@ -207,6 +212,9 @@ class GmresState:
         # straight into V and the graph input (no copies, no host step)
         self._op = operator
         self._fast = False
+        # the stream all of this solve's device work goes to (a server sets
+        # it); None: the current stream at each call
+        self.stream = None
         if self._hip:
             from . import _native as N
             nd = ctypes.c_int64(0)
@@ -233,9 +241,29 @@ class GmresState:
         return (self.m + 5) * self.n * 8
 
     # -- device side --------------------------------------------------------
+    def _ctx(self):
+        return (torch.cuda.stream(self.stream) if self.stream is not None
+                else contextlib.nullcontext())
+
     def issue(self):
-        """Enqueue the next step's device work; return the device tensor
-        whose host value ``consume`` needs."""
+        """Enqueue the next step's device work (on ``stream`` when set, else
+        the current stream); return the device tensor whose host value
+        ``consume`` needs."""
+        if self.phase == 'arnoldi' and self._fast:
+            # graph launch + CGS2 in one native call; the next basis vector
+            # goes straight into V[j+1] and the graph input
+            from . import _native as N
+            j = self.j
+            sp = self.stream.cuda_stream if self.stream is not None else N.stream_ptr()
+            N.call('tct_arnoldi_step', self._gexec, j + 1, self.n, self.n, self._pV, self._pgy,
+                   self._ph, self._psc, self._nsc, self._pV + (j + 1) * self._row_bytes,
+                   self._pgx, sp)
+            self._op.n_matvec[self.inner_prec] += 1
+            return self._h[:j + 2]
+        with self._ctx():
+            return self._issue()
+
+    def _issue(self):
         if self.phase == 'norm_b':
             return torch.linalg.vector_norm(self.b).reshape(1)
         if self.phase == 'residual':
@@ -257,16 +285,8 @@ class GmresState:
                 self.apply(self.x, self.r, self._res_prec)
                 torch.sub(self.b, self.r, out=self.r)
             return torch.linalg.vector_norm(self.r).reshape(1)
-        # arnoldi step j
+        # arnoldi step j (without the operator's graph)
         j = self.j
-        if self._fast:
-            from . import _native as N
-            op = self._op
-            op.replay(self.inner_prec)
-            N.call('tct_cgs2', j + 1, self.n, self.n, N.ptr(self.V), N.ptr(op.gy),
-                   N.ptr(self._h), N.ptr(self._scratch), self._scratch.shape[0],
-                   N.ptr(self.V[j + 1]), N.ptr(op.gx), N.stream_ptr())
-            return self._h[:j + 2]
         self.apply(self.V[j], self.w, self.inner_prec)
         if self._hip:
             from . import _native as N
@@ -283,7 +303,16 @@ class GmresState:
 
     # -- host side ----------------------------------------------------------
     def consume(self, vals):
+        """Take the host values of the last issued step.  Device work this
+        enqueues (basis vector, solution update) runs on ``stream`` when
+        set; a mid-cycle Arnoldi step of the graph path enqueues none."""
         vals = np.asarray(vals, dtype=np.float64).reshape(-1)
+        if self.phase == 'arnoldi' and self._fast:
+            return self._consume_arnoldi(vals)
+        with self._ctx():
+            return self._consume(vals)
+
+    def _consume(self, vals):
         if self.phase == 'norm_b':
             self.bnorm = float(vals[0])
             if self.bnorm == 0.0 or not math.isfinite(self.bnorm):
@@ -323,7 +352,15 @@ class GmresState:
             self._fast = (self._hip and self._op is not None and
                           self._op.has_graph(self.inner_prec))
             if self._fast:
-                self._op.gx.copy_(self.V[0])
+                op = self._op
+                op.gx.copy_(self.V[0])
+                # addresses of the native Arnoldi step
+                self._gexec = op.graph_exec(self.inner_prec)
+                self._pV, self._pgx, self._pgy = (self.V.data_ptr(), op.gx.data_ptr(),
+                                                  op.gy.data_ptr())
+                self._ph, self._psc = self._h.data_ptr(), self._scratch.data_ptr()
+                self._nsc = self._scratch.shape[0]
+                self._row_bytes = self.V.stride(0) * self.V.element_size()
             # Givens state in Python floats (the per-step host work is a few
             # dozen scalar operations); R's columns kept as lists
             self.Rcols = []
@@ -336,7 +373,9 @@ class GmresState:
             self.j = 0
             self.phase = 'arnoldi'
             return False
-        # arnoldi
+        return self._consume_arnoldi(vals)
+
+    def _consume_arnoldi(self, vals):
         j = self.j
         col = vals.tolist()
         hnext = col[j + 1]
@@ -365,6 +404,12 @@ class GmresState:
             return False
         # close the cycle: x += V_k y
         self._expect = est / self.bnorm if math.isfinite(est) else float('inf')
+        with self._ctx():
+            self._close_cycle(est, k)
+        self.phase = 'residual'
+        return False
+
+    def _close_cycle(self, est, k):
         if math.isfinite(est):
             R = np.zeros((k, k))
             for jj, cj in enumerate(self.Rcols):
@@ -392,8 +437,6 @@ class GmresState:
                 self.d.zero_()
         elif self.d is not None:
             self.d.zero_()
-        self.phase = 'residual'
-        return False
 
     def _finish(self, status):
         self.done = True
