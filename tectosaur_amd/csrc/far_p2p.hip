@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
 
     for (int t = t0; t < t1; ++t) {
         const double os0 = src_org[3 * t], os1 = src_org[3 * t + 1], os2 = src_org[3 * t + 2];
-        float xs[P][3], acc[P][NO];
+        float xs[P][3], acc[P][NO], am[P][n_mterm_arr<KID>()];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             xs[p][0] = xl[p][0] - (float)(os0 - oo[p][0]);
@@ -72,18 +72,22 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
             xs[p][2] = xl[p][2] - (float)(os2 - oo[p][2]);
 #pragma unroll
             for (int a = 0; a < NO; ++a) acc[p][a] = 0.0f;
+#pragma unroll
+            for (int a = 0; a < n_mterm_arr<KID>(); ++a) am[p][a] = 0.0f;
         }
         const float *r = rec + (size_t)t * G * RW;
 #pragma unroll 4
         for (int j = 0; j < G; ++j) {
             const float *q = r + j * RW;
 #pragma unroll
-            for (int p = 0; p < P; ++p) pair_eval<KID, float>(xs[p], m[p], q, k, acc[p]);
+            for (int p = 0; p < P; ++p) pair_eval<KID, float>(xs[p], m[p], q, k, acc[p], am[p]);
         }
 #pragma unroll
-        for (int p = 0; p < P; ++p)
+        for (int p = 0; p < P; ++p) {
+            fold_mterm<KID, float>(m[p], k, acc[p], am[p]);
 #pragma unroll
             for (int a = 0; a < NO; ++a) acc64[p][a] += (double)acc[p][a];
+        }
     }
 
     double *out = partial + (size_t)c * NO * n_obs_pad;
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
     const int t0 = (int)(((long)n_tiles * c) / chunks);
     const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
 
-    double x[P][3], m[P][3], acc[P][NO];
+    double x[P][3], m[P][3], acc[P][NO], am[P][n_mterm_arr<KID>()];
     int idx[P];
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -128,6 +132,8 @@ __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
         }
 #pragma unroll
         for (int a = 0; a < NO; ++a) acc[p][a] = 0.0;
+#pragma unroll
+        for (int a = 0; a < n_mterm_arr<KID>(); ++a) am[p][a] = 0.0;
     }
     const double *r = rec + (size_t)t0 * G * RW;
     const int nj = (t1 - t0) * G;
@@ -135,8 +141,10 @@ __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
     for (int j = 0; j < nj; ++j) {
         const double *q = r + (size_t)j * RW;
 #pragma unroll
-        for (int p = 0; p < P; ++p) pair_eval<KID, double>(x[p], m[p], q, k, acc[p]);
+        for (int p = 0; p < P; ++p) pair_eval<KID, double>(x[p], m[p], q, k, acc[p], am[p]);
     }
+#pragma unroll
+    for (int p = 0; p < P; ++p) fold_mterm<KID, double>(m[p], k, acc[p], am[p]);
     double *out = partial + (size_t)c * NO * n_obs_pad;
 #pragma unroll
     for (int p = 0; p < P; ++p) {

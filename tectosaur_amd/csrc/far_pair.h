@@ -35,6 +35,14 @@ template <int KID>
 constexpr int n_out() { return (KID == TCT_RA || KID == TCT_RH) ? 9 : 3; }
 template <int KID>
 constexpr bool needs_obs_normal() { return KID == TCT_A || KID == TCT_H || KID == TCT_RA; }
+// Terms whose vector is the observation normal m are summed as per-lane
+// scalars (am) and multiplied by m once per source tile (fold_mterm), not
+// per pair: A has one (-CsT0/CsT1 (D.s)/r^3), H two (e0 / r^3 and
+// 3 CsH1 (n.D)(s.D)/r^5).
+template <int KID>
+constexpr int n_mterm() { return KID == TCT_H ? 2 : (KID == TCT_A ? 1 : 0); }
+template <int KID>
+constexpr int n_mterm_arr() { return n_mterm<KID>() > 0 ? n_mterm<KID>() : 1; }
 
 // One (observation point, source point) pair, accumulate into acc[n_out].
 // x: observation point (already shifted into the source tile frame);
@@ -48,7 +56,7 @@ constexpr bool needs_obs_normal() { return KID == TCT_A || KID == TCT_H || KID =
 //            P = w J CsRH0 (CsRH1 tr(S) I + CsRH2 S^T - 2 S).
 template <int KID, typename T>
 __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *__restrict__ q,
-                                          const KConst<T> &k, T *acc) {
+                                          const KConst<T> &k, T *acc, T *am) {
     const T D0 = q[0] - x[0], D1 = q[1] - x[1], D2 = q[2] - x[2];
     const T r2 = D0 * D0 + D1 * D1 + D2 * D2;
     const T invr = rsq_masked(r2, k.eps2);
@@ -59,9 +67,14 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
         const T ds = D0 * s0 + D1 * s1 + D2 * s2;
         const T a = k.c0 * invr;
         const T b = invr * w * ds;
-        acc[0] += a * s0 + b * D0;
-        acc[1] += a * s1 + b * D1;
-        acc[2] += a * s2 + b * D2;
+        // one FMA per term: a sum of products added to acc would cost a
+        // multiply and an add more per component
+        acc[0] += a * s0;
+        acc[0] += b * D0;
+        acc[1] += a * s1;
+        acc[1] += b * D1;
+        acc[2] += a * s2;
+        acc[2] += b * D2;
     } else if constexpr (KID == TCT_T) {
         // T: A(-(n.D) s + (n x D) x s) - C (n.D)(D.s) D with
         // (n x D) x s = D (n.s) - n (D.s); strengths scaled by CsT1,
@@ -75,9 +88,15 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
         const T be = -al * rn;
         const T ga = -al * ds;
         const T de = al * q[9] - q3 * w * rn * ds;
-        acc[0] += be * s0 + ga * n0 + de * D0;
-        acc[1] += be * s1 + ga * n1 + de * D1;
-        acc[2] += be * s2 + ga * n2 + de * D2;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const T sd = d == 0 ? s0 : (d == 1 ? s1 : s2);
+            const T nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
+            const T Dd = d == 0 ? D0 : (d == 1 ? D1 : D2);
+            acc[d] += be * sd;
+            acc[d] += ga * nd;
+            acc[d] += de * Dd;
+        }
     } else if constexpr (KID == TCT_A) {
         // A: the adjoint form with the observation normal m.
         const T s0 = q[6], s1 = q[7], s2 = q[8];
@@ -87,11 +106,14 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
         const T ds = D0 * s0 + D1 * s1 + D2 * s2;
         const T sm = m[0] * s0 + m[1] * s1 + m[2] * s2;
         const T be = al * rn;
-        const T ga = -al * ds;
         const T de = al * sm + q3 * w * rn * ds;
-        acc[0] += be * s0 + ga * m[0] + de * D0;
-        acc[1] += be * s1 + ga * m[1] + de * D1;
-        acc[2] += be * s2 + ga * m[2] + de * D2;
+        acc[0] += be * s0;
+        acc[0] += de * D0;
+        acc[1] += be * s1;
+        acc[1] += de * D1;
+        acc[2] += be * s2;
+        acc[2] += de * D2;
+        am[0] += q3 * ds;   // m term: -al (D.s) m
     } else if constexpr (KID == TCT_H) {
         // H with strengths scaled by CsH0; k = (CsH1, CsH3, 3 CsH1, nu, 3 nu),
         // e0 = CsH2 (n.s), e1 = 3 CsH1 (n.s).
@@ -105,14 +127,23 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
         const T mn = m[0] * n0 + m[1] * n1 + m[2] * n2;
         const T wa = w * a;
         const T wbc = w * b * c;
-        const T MT = Q * (q[9] + k.c2 * wa * c);
+        // m term Q (e0 + c2 wa c): two lane scalars, folded per tile
+        am[0] += Q * q[9];
+        am[1] += Q * (wa * c);
         const T NT = Q * (k.c0 * sm + k.c1 * wbc);
-        const T inner = k.c3 * sm - T(5) * wbc;
-        const T DT = Q * w * (q[10] * b + k.c1 * c * mn + T(3) * a * inner);
+        // 3 (c3 sm - 5 wbc) with k.c4 = 3 c3
+        const T inner = k.c4 * sm - T(15) * wbc;
+        const T DT = Q * w * (q[10] * b + c * (k.c1 * mn) + a * inner);
         const T ST = Q * (k.c4 * wa * b + k.c0 * mn);
-        acc[0] += n0 * NT + m[0] * MT + D0 * DT + s0 * ST;
-        acc[1] += n1 * NT + m[1] * MT + D1 * DT + s1 * ST;
-        acc[2] += n2 * NT + m[2] * MT + D2 * DT + s2 * ST;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const T sd = d == 0 ? s0 : (d == 1 ? s1 : s2);
+            const T nd = d == 0 ? n0 : (d == 1 ? n1 : n2);
+            const T Dd = d == 0 ? D0 : (d == 1 ? D1 : D2);
+            acc[d] += nd * NT;
+            acc[d] += Dd * DT;
+            acc[d] += sd * ST;
+        }
     } else if constexpr (KID == TCT_RT) {
         // RT: invr e + (CsRT0/CsRT1) invr^3 (D.e) D + invr^3 (n.D) s
         const T i3 = invr * w;
@@ -120,9 +151,13 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
         const T nD = D0 * q[3] + D1 * q[4] + D2 * q[5];
         const T cd = k.c0 * i3 * De;
         const T cs = i3 * nD;
-        acc[0] += invr * q[9] + cd * D0 + cs * q[6];
-        acc[1] += invr * q[10] + cd * D1 + cs * q[7];
-        acc[2] += invr * q[11] + cd * D2 + cs * q[8];
+        const T Dk[3] = {D0, D1, D2};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            acc[d] += invr * q[9 + d];
+            acc[d] += cd * Dk[d];
+            acc[d] += cs * q[6 + d];
+        }
     } else if constexpr (KID == TCT_RA) {
         // RA moments: v1 += s / r, v2 += (D.s) D / r^3, v3 += (m.D) s / r^3;
         // the constants and the observation curls are applied by the scatter.
@@ -150,12 +185,31 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
         for (int kk = 0; kk < 3; ++kk) {
             const T dq = qd * Dk[kk];
 #pragma unroll
-            for (int mm = 0; mm < 3; ++mm)
-                acc[kk * 3 + mm] += invr * q[12 + kk * 3 + mm] + dq * u[mm];
+            for (int mm = 0; mm < 3; ++mm) {
+                acc[kk * 3 + mm] += invr * q[12 + kk * 3 + mm];
+                acc[kk * 3 + mm] += dq * u[mm];
+            }
         }
     }
 }
 
+// acc += m * (the m-term scalars), am reset; a no-op for kernels without.
+template <int KID, typename T>
+__device__ __forceinline__ void fold_mterm(const T m[3], const KConst<T> &k, T *acc, T *am) {
+    if constexpr (KID == TCT_H) {
+        const T f = am[0] + k.c2 * am[1];
+        acc[0] += m[0] * f;
+        acc[1] += m[1] * f;
+        acc[2] += m[2] * f;
+        am[0] = am[1] = T(0);
+    } else if constexpr (KID == TCT_A) {
+        const T f = -k.c0 * am[0];
+        acc[0] += m[0] * f;
+        acc[1] += m[1] * f;
+        acc[2] += m[2] * f;
+        am[0] = T(0);
+    }
+}
 
 template <typename T>
 inline KConst<T> kernel_consts(int kid, const double *params, double eps2) {

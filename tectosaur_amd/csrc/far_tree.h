@@ -381,9 +381,10 @@ __device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__r
                 __builtin_amdgcn_sched_barrier(0);
                 static_for<I0, I1>([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
-                    h[aa] += M[(4 + 3 * aa + 0) * TT + I] * tk(ic, 0) +
-                             M[(4 + 3 * aa + 1) * TT + I] * tk(ic, 1) +
-                             M[(4 + 3 * aa + 2) * TT + I] * tk(ic, 2);
+                    // one FMA per term (a sum added to h costs two more ops)
+                    h[aa] += M[(4 + 3 * aa + 0) * TT + I] * tk(ic, 0);
+                    h[aa] += M[(4 + 3 * aa + 1) * TT + I] * tk(ic, 1);
+                    h[aa] += M[(4 + 3 * aa + 2) * TT + I] * tk(ic, 2);
                 });
             });
         });
@@ -484,8 +485,10 @@ __global__ __launch_bounds__(64) void tree_eval_kernel(
             for (int d = 0; d < 3; ++d)
                 xs[d] = ABS ? x[d] : x[d] - (T)(src_org[3 * t + d] - oo[d]);
             const T *r = rec + (size_t)t * G * RW;
+            T am[n_mterm_arr<KID>()] = {};
 #pragma unroll 2
-            for (int j = 0; j < G; ++j) pair_eval<KID, T>(xs, mn, r + j * RW, k, acc);
+            for (int j = 0; j < G; ++j) pair_eval<KID, T>(xs, mn, r + j * RW, k, acc, am);
+            fold_mterm<KID, T>(mn, k, acc, am);
         } else {
             const T *Mg = moments + (size_t)code * NM;
             constexpr int NL = (NM + G - 1) / G;
