@@ -219,7 +219,8 @@ int tct_tree_reduce(int kernel, int n_obs_tiles, int n_obs_pad, const int *seg_o
  *   check0: skip quadrature points with r2 == 0 (the "Z" variant);
  *   out [n_pairs][81] f64.
  * lanes_per_pair (1, 4, 16 or 64) sets how many lanes share one pair's
- * quadrature points; 0 picks from nq. */
+ * quadrature points; 0 picks from nq and n_pairs (the fewest lanes that
+ * still fill the device). */
 int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, const double *qw,
                         const double *pts, const int32_t *tris, int64_t n_pairs,
                         const int32_t *pairs, int pair_width, int check0,

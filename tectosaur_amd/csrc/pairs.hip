@@ -26,24 +26,59 @@
 
 namespace tct {
 
-template <int KID>
-__device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, const double *q,
-                                                 double w, int check0, const ElasticConsts &c,
-                                                 double *acc) {
-    const double ox = q[0], oy = q[1], sx = q[2], sy = q[3];
-    const double po[3] = {1.0 - ox - oy, ox, oy};
-    const double ps[3] = {1.0 - sx - sy, sx, sy};
-    // basis-weighted vertex sums, as pts_from_basis (integral_primitives.cl:121-128)
-    double D[3];
+// Per-pair geometry of D = y(sx, sy) - x(ox, oy) for the P1 maps
+// x = sum_b phi_b(ox, oy) v_b (pts_from_basis, integral_primitives.cl:121-128)
+// written in edge form: D = b + sx se0 + sy se1 + ox oe0 + oy oe1 with
+// b = s_v0 - o_v0, se = s_v{1,2} - s_v0, oe = -(o_v{1,2} - o_v0): four FMAs
+// per component and quadrature point instead of two basis-weighted sums.
+// Used where no quadrature point can coincide (check0 off: near pairs).
+struct PairGeom {
+    double b[3], se[2][3], oe[2][3];
+};
+
+__device__ __forceinline__ void pair_geom(const Tri &ot, const Tri &st, PairGeom &g) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double x = po[0] * ot.v[0][d] + po[1] * ot.v[1][d] + po[2] * ot.v[2][d];
-        const double y = ps[0] * st.v[0][d] + ps[1] * st.v[1][d] + ps[2] * st.v[2][d];
-        D[d] = y - x;
+        g.b[d] = st.v[0][d] - ot.v[0][d];
+        g.se[0][d] = st.v[1][d] - st.v[0][d];
+        g.se[1][d] = st.v[2][d] - st.v[0][d];
+        g.oe[0][d] = ot.v[0][d] - ot.v[1][d];
+        g.oe[1][d] = ot.v[0][d] - ot.v[2][d];
+    }
+}
+
+template <int KID, bool CHECK0>
+__device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, const PairGeom &g,
+                                                 const double *q, double w,
+                                                 const ElasticConsts &c, double *acc) {
+    const double ox = q[0], oy = q[1], sx = q[2], sy = q[3];
+    double D[3];
+    if constexpr (CHECK0) {
+        // pairs whose rule can hit r = 0 exactly: the reference's
+        // basis-weighted sums, so coincident points give D = 0 bit for bit
+        // and are skipped (the edge form leaves a rounding residue there)
+        const double po[3] = {1.0 - ox - oy, ox, oy};
+        const double ps[3] = {1.0 - sx - sy, sx, sy};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double x = po[0] * ot.v[0][d] + po[1] * ot.v[1][d] + po[2] * ot.v[2][d];
+            const double y = ps[0] * st.v[0][d] + ps[1] * st.v[1][d] + ps[2] * st.v[2][d];
+            D[d] = y - x;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double t = fma(sx, g.se[0][d], g.b[d]);
+            t = fma(sy, g.se[1][d], t);
+            t = fma(ox, g.oe[0][d], t);
+            D[d] = fma(oy, g.oe[1][d], t);
+        }
     }
     const double r2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
-    if (check0 && r2 == 0.0) return;
+    if (CHECK0 && r2 == 0.0) return;
     if constexpr (KID <= TCT_H) {
+        const double po[3] = {1.0 - ox - oy, ox, oy};
+        const double ps[3] = {1.0 - sx - sy, sx, sy};
         double K[9];
         tensor<KID>(D, r2, ot.n, st.n, c, K);
 #pragma unroll
@@ -62,12 +97,15 @@ __device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, c
         const double wi = w * invr;
         const double wi3 = wi * invr * invr;
         acc[0] += wi;
+        const double e[3] = {wi3 * D[0], wi3 * D[1], wi3 * D[2]};
 #pragma unroll
         for (int p = 0; p < 3; ++p)
 #pragma unroll
-            for (int t = p; t < 3; ++t) acc[1 + sym6(p, t)] += wi3 * D[p] * D[t];
+            for (int t = p; t < 3; ++t) acc[1 + sym6(p, t)] += e[p] * D[t];
     } else {
         // RT (weights follow the obs basis, normal = src) or RA (src basis, obs normal).
+        const double po[3] = {1.0 - ox - oy, ox, oy};
+        const double ps[3] = {1.0 - sx - sy, sx, sy};
         const double invr = rsqrt(r2);
         const double invr3 = invr * invr * invr;
         const double *nn = KID == TCT_RT ? st.n : ot.n;
@@ -136,13 +174,13 @@ __device__ __forceinline__ double final_value(const double *acc, const Tri &ot, 
     }
 }
 
-template <int KID, int GL>
+template <int KID, int GL, bool CHECK0>
 __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__restrict__ qx,
                                                     const double *__restrict__ qw,
                                                     const double *__restrict__ pts,
                                                     const int32_t *__restrict__ tris, int n_pairs,
                                                     const int32_t *__restrict__ pairs,
-                                                    int pair_width, int check0, ElasticConsts c,
+                                                    int pair_width, ElasticConsts c,
                                                     double *__restrict__ out) {
     constexpr int NA = n_acc<KID>();
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -161,13 +199,15 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
     Tri ot, st;
     load_tri(ot, pts, tris, pp[0], oc, 0);
     load_tri(st, pts, tris, pp[1], sc, flip);
+    PairGeom g;
+    pair_geom(ot, st, g);
 
     double acc[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
     if (live)
         for (int iq = sub; iq < nq; iq += GL)
-            accumulate_point<KID>(ot, st, qx + 4 * (size_t)iq, qw[iq], check0, c, acc);
+            accumulate_point<KID, CHECK0>(ot, st, g, qx + 4 * (size_t)iq, qw[iq], c, acc);
     if constexpr (GL > 1) {
 #pragma unroll
         for (int k = 0; k < NA; ++k) {
@@ -225,8 +265,12 @@ static void launch_pairs(hipStream_t st, int nq, const double *qx, const double 
                          const ElasticConsts &c, double *out) {
     const int64_t threads = (int64_t)n_pairs * GL;
     const unsigned nb = (unsigned)ceil_div(threads, 256);
-    hipLaunchKernelGGL((pairs_kernel<KID, GL>), dim3(nb), dim3(256), 0, st, nq, qx, qw, pts, tris,
-                       n_pairs, pairs, pair_width, check0, c, out);
+    if (check0)
+        hipLaunchKernelGGL((pairs_kernel<KID, GL, true>), dim3(nb), dim3(256), 0, st, nq, qx, qw,
+                           pts, tris, n_pairs, pairs, pair_width, c, out);
+    else
+        hipLaunchKernelGGL((pairs_kernel<KID, GL, false>), dim3(nb), dim3(256), 0, st, nq, qx, qw,
+                           pts, tris, n_pairs, pairs, pair_width, c, out);
 }
 
 template <int KID>
@@ -260,7 +304,23 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
                   "pairs: null pointer");
     if (n_pairs == 0) return TCT_OK;
     int gl = lanes_per_pair;
-    if (gl == 0) gl = nq >= 1024 ? 64 : (nq >= 128 ? 16 : (nq >= 24 ? 4 : 1));
+    if (gl == 0) {
+        // Fewest lanes per pair that still give enough wavefronts, at most
+        // what the rule size can use.  One lane per pair reads the rule
+        // through wave-uniform scalar loads: on 1.7M near pairs (4,096-point
+        // rule, no r = 0 test) it measured 1.6x faster than 64 lanes, and
+        // it stays ahead down to ~10k wavefronts; the classes with the
+        // r = 0 test (coincident, adjacent) keep gaining from more lanes up
+        // to ~16k-46k wavefronts (64-180 per CU).
+        const int gmax = nq >= 1024 ? 64 : (nq >= 128 ? 16 : (nq >= 24 ? 4 : 1));
+        const int64_t min_waves = check0 ? 16384 : 4096;
+        gl = gmax;
+        for (int g = 1; g < gmax; g *= 4)
+            if (n_pairs * g >= 64LL * min_waves) {
+                gl = g;
+                break;
+            }
+    }
     TCT_CHECK_ARG(gl == 1 || gl == 4 || gl == 16 || gl == 64, "pairs: lanes_per_pair %d", gl);
     TCT_CHECK_ARG(n_pairs * gl < (1LL << 31), "pairs: n_pairs * lanes_per_pair >= 2^31");
     const ElasticConsts c = elastic_consts(params_host);
