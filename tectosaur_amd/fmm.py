@@ -224,10 +224,13 @@ class TreeFarfield:
         self.seg_off = N.upload(seg_off, None, dev)
         self.chunks = 1  # the reduced output has one chunk
         # upward pass: one launch per level, only up to the highest level any
-        # list refers to.  Two one-launch forms measured slower at 4k-32k
+        # list refers to.  Three one-launch forms measured slower at 4k-32k
         # triangles: the block that finishes a parent's last child going on
-        # to the parent (device-scope fences between levels, 1.4x), and one
-        # workgroup forming the small top levels in turn (3x at 4k)
+        # to the parent (device-scope fences between levels, 1.4x), one
+        # workgroup forming the small top levels node after node (3x at 4k),
+        # and one workgroup forming them with every node of a level at once
+        # (180 us for the top five levels at 8k against 55 us per level: four
+        # waves cannot hide the translation's LDS and FP64 latency)
         used = lst[lst >= 0]
         self.max_level = int(self.tree.level[used].max()) if used.size else 0
         self._moments = {}
