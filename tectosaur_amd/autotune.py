@@ -33,6 +33,7 @@ def _size_class(n):
 
 
 _ARCH = {}
+_ARCH_LOCK = threading.Lock()   # its own lock: _key is called with _LOCK held
 
 
 def device_arch(device):
@@ -42,7 +43,7 @@ def device_arch(device):
     idx = torch.device(device).index
     if idx is None:
         idx = torch.cuda.current_device()
-    with _LOCK:
+    with _ARCH_LOCK:
         arch = _ARCH.get(idx)
         if arch is None:
             arch = torch.cuda.get_device_properties(idx).gcnArchName.split(':')[0]
@@ -118,8 +119,9 @@ def tune(engine, prec, reps=3, save=True, verbose=False):
     engine.tile_cfg[prec] = best[0]
     if save:
         t = _load()
+        key = _key(engine.K.name, prec, engine.n_obs_pad, engine.n_src_pad, engine.device)
         with _LOCK:
-            t[_key(engine.K.name, prec, engine.n_obs_pad, engine.n_src_pad, engine.device)] = {
+            t[key] = {
                 'block': best[0].block, 'obs_per_lane': best[0].obs_per_lane,
                 'src_chunks': best[0].src_chunks, 'seconds': best[1]}
             try:
