@@ -26,7 +26,10 @@ def find_near_pairs(ca, ra, cb, rb, threshold):
     |c_i - c_j|^2 <= ((r_i + r_j) * threshold)^2, sorted by (i, j)."""
     ca, ra, cb, rb = (np.ascontiguousarray(x, dtype=np.float64) for x in (ca, ra, cb, rb))
     n = ctypes.c_int64(0)
-    cap = max(16, 64 * ca.shape[0])
+    # room for 256 pairs per point (meshes at the usual thresholds have
+    # ~90): an overflow runs the search again with the exact count; pages
+    # of the buffer that are never written are never committed
+    cap = max(16, 256 * ca.shape[0])
     while True:
         out = np.empty((cap, 2), dtype=np.int64)
         N.call('tct_find_near_pairs', ca.shape[0], ca.ctypes.data, ra.ctypes.data, cb.shape[0],
@@ -38,8 +41,12 @@ def find_near_pairs(ca, ra, cb, rb, threshold):
 
 
 def find_close_or_touching(a_pts, a_tris, b_pts, b_tris, threshold):
-    return find_near_pairs(*get_tri_centroids_rs(a_pts, a_tris),
-                           *get_tri_centroids_rs(b_pts, b_tris), threshold)
+    ca, ra = get_tri_centroids_rs(a_pts, a_tris)
+    if (a_pts is b_pts or np.array_equal(a_pts, b_pts)) and np.array_equal(a_tris, b_tris):
+        cb, rb = ca, ra
+    else:
+        cb, rb = get_tri_centroids_rs(b_pts, b_tris)
+    return find_near_pairs(ca, ra, cb, rb, threshold)
 
 
 def split_adjacent_close(close_pairs, tris_A, tris_B):

@@ -40,24 +40,26 @@ def n_terms(p):
     return (p + 1) * (p + 2) * (p + 3) // 6
 
 
-def _tile_bounds(pts, n_pad, lo_tiles, hi_tiles):
+def _tile_bounds(pts, n_pad, lo_tiles, hi_tiles, box=None):
     """Centre (bounding-box midpoint) and radius of point ranges given in
-    tiles [lo, hi); padding points beyond pts.shape[0] are excluded."""
+    tiles [lo, hi); padding points beyond pts.shape[0] are excluded.  The
+    ranges must be consecutive and cover [lo[0], hi[-1]).  ``box`` (bmin,
+    bmax) may give the ranges' bounding boxes when already known."""
     n = pts.shape[0]
     lo = np.minimum(lo_tiles * GROUP, n)
     hi = np.minimum(hi_tiles * GROUP, n)
     cnt = hi - lo
     if np.any(cnt <= 0):
         raise ValueError('empty tree node')
-    bmin = np.minimum.reduceat(pts, lo, axis=0)
-    bmax = np.maximum.reduceat(pts, lo, axis=0)
-    # reduceat with lo as boundaries only works when ranges are contiguous
-    # and cover [lo[0], n); the callers pass consecutive ranges.
+    if box is None:
+        box = (np.minimum.reduceat(pts, lo, axis=0), np.maximum.reduceat(pts, lo, axis=0))
+    bmin, bmax = box
     c = 0.5 * (bmin + bmax)
     owner = np.repeat(np.arange(lo.shape[0]), cnt)
-    d = np.linalg.norm(pts[lo[0]:hi[-1]] - c[owner], axis=1)
-    r = np.maximum.reduceat(d, lo - lo[0])
-    return c, r
+    diff = pts[lo[0]:hi[-1]] - c[owner]
+    d2 = np.einsum('ij,ij->i', diff, diff)
+    r = np.sqrt(np.maximum.reduceat(d2, lo - lo[0]))
+    return c, r, box
 
 
 class SourceTree:
@@ -79,11 +81,19 @@ class SourceTree:
         self.levels = []
         cr, lo_all, hi_all, lev_all = [], [], [], []
         off = 0
+        box = None
         for lvl, cnt in enumerate(counts):
             span = 1 << lvl
             lo = np.arange(cnt) * span
             hi = np.minimum(lo + span, self.n_tiles)
-            c, r = _tile_bounds(pts, n_pad, lo, hi)
+            if box is not None:
+                # a parent's box is the union of its children's (exact)
+                bmin, bmax = box
+                if bmin.shape[0] % 2:
+                    bmin = np.concatenate([bmin, bmin[-1:]])
+                    bmax = np.concatenate([bmax, bmax[-1:]])
+                box = (np.minimum(bmin[0::2], bmin[1::2]), np.maximum(bmax[0::2], bmax[1::2]))
+            c, r, box = _tile_bounds(pts, n_pad, lo, hi, box)
             cr.append(np.concatenate([c, np.maximum(r, rmin)[:, None]], axis=1))
             lo_all.append(lo)
             hi_all.append(hi)
@@ -149,7 +159,7 @@ def interaction_lists(obs_cr, tree, mac, leaf_level=0):
 def obs_tile_cr(pts, n_pad):
     n_tiles = n_pad // GROUP
     lo = np.arange(n_tiles)
-    c, r = _tile_bounds(np.ascontiguousarray(pts, dtype=np.float64), n_pad, lo, lo + 1)
+    c, r, _ = _tile_bounds(np.ascontiguousarray(pts, dtype=np.float64), n_pad, lo, lo + 1)
     return np.concatenate([c, r[:, None]], axis=1)
 
 

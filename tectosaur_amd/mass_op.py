@@ -44,17 +44,20 @@ class MassOp:
         jac = np.linalg.norm(un, axis=1)
         bf = np.einsum('q,qa,qb->ab', qw, basis, basis).reshape(-1)
         n = tris.shape[0]
-        i = np.arange(n)[:, None, None, None]
-        b1 = np.arange(3)[None, :, None, None]
-        b2 = np.arange(3)[None, None, :, None]
-        d = np.arange(tensor_dim)[None, None, None, :]
-        rows = ((i * 3 + b1) * tensor_dim + d) + 0 * b2
-        cols = ((i * 3 + b2) * tensor_dim + d) + 0 * b1
-        vals = (jac[:, None, None, None] * bf.reshape(3, 3)[None, :, :, None]) + 0 * d
-        n_rows = n * 3 * tensor_dim
+        td = tensor_dim
+        n_rows = n * 3 * td
         self.shape = (n_rows, n_rows)
-        self.mat = scipy.sparse.csr_matrix((vals.reshape(-1), (rows.reshape(-1),
-                                                               cols.reshape(-1))),
+        # row (tri i, basis b1, component d) holds the three entries
+        # jac_i bf[b1][b2] at columns (i, b2, d), b2 = 0, 1, 2 (already in
+        # column order): the CSR arrays are formed directly, no sort
+        r = np.arange(n_rows)
+        i = r // (3 * td)
+        b1 = (r // td) % 3
+        d = r % td
+        indices = (3 * i[:, None] + np.arange(3)[None, :]) * td + d[:, None]
+        data = jac[i][:, None] * bf.reshape(3, 3)[b1]
+        indptr = np.arange(0, 3 * n_rows + 1, 3)
+        self.mat = scipy.sparse.csr_matrix((data.reshape(-1), indices.reshape(-1), indptr),
                                            shape=self.shape)
         self._dev = None
 

@@ -45,11 +45,20 @@ def resolve_ea_rotation(tris, ea):
     return np.stack([ea[:, 0], ea[:, 1], oc, sc, flip.astype(np.int64)], axis=1)
 
 
+def _is_identity(subset):
+    s = np.asarray(subset)
+    return s.ndim == 1 and (s.size == 0 or (s[0] == 0 and s[-1] == s.size - 1 and
+                                            bool(np.all(np.diff(s) == 1))))
+
+
 def to_tri_space(dof_indices, obs_subset, src_subset):
     di = np.asarray(dof_indices, dtype=np.int64)
     out = di.copy()
-    out[:, 0] = obs_subset[di[:, 0]]
-    out[:, 1] = src_subset[di[:, 1]]
+    # identity subsets (the whole mesh) map positions to themselves
+    if not _is_identity(obs_subset):
+        out[:, 0] = obs_subset[di[:, 0]]
+    if not _is_identity(src_subset):
+        out[:, 1] = src_subset[di[:, 1]]
     return out
 
 

@@ -62,8 +62,11 @@ def farfield_plan(nq_far, pts, tris, obs_subset, src_subset, curl):
         c = np.stack([(-e2 + e1) / J, e2 / J, -e1 / J], axis=1)
         return np.ascontiguousarray(c.reshape((-1, 9)))
 
+    same = (obs_subset.shape == src_subset.shape and
+            np.array_equal(obs_subset, src_subset))
     o_order, o_pts, o_ns, o_wj = side(obs_subset)
-    s_order, s_pts, s_ns, s_wj = side(src_subset)
+    # the full operator (obs set == src set) packs one point set twice
+    s_order, s_pts, s_ns, s_wj = (o_order, o_pts, o_ns, o_wj) if same else side(src_subset)
     n_obs = obs_subset.shape[0]
     pt_of_tri = np.empty(n_obs, dtype=np.int32)
     pt_of_tri[o_order] = np.arange(n_obs, dtype=np.int32)
@@ -73,7 +76,7 @@ def farfield_plan(nq_far, pts, tris, obs_subset, src_subset, curl):
                 extents=(group_extent(o_pts), group_extent(s_pts)))
     if curl:
         plan['curl_src'] = curls(src_subset)
-        plan['curl_obs'] = curls(obs_subset)
+        plan['curl_obs'] = plan['curl_src'] if same else curls(obs_subset)
     return plan
 
 
