@@ -97,18 +97,29 @@ def candidates(n_obs_pad, n_src_pad, n_cu=256):
     return out
 
 
+def _run_and_reduce(engine, prec, cfg):
+    """The N-body launch and what a matvec then pays for its source chunks:
+    one read of every chunk's partial sums (each chunk writes n_out x
+    n_obs_pad doubles, so many chunks cost traffic at small sizes)."""
+    out = engine.run(prec, cfg)
+    if cfg.src_chunks > 1:
+        out[:cfg.src_chunks * engine.n_out * engine.n_obs_pad].view(cfg.src_chunks, -1).sum(0)
+    return out
+
+
 def tune(engine, prec, reps=3, save=True, verbose=False):
     """Time every candidate launch shape on ``engine`` (a FarfieldEngine with
-    strengths already set) and store the fastest.  Returns (cfg, seconds)."""
+    strengths already set), the N-body and the reduction of its chunk
+    partials together, and store the fastest.  Returns (cfg, seconds)."""
     n_cu = torch.cuda.get_device_properties(engine.device).multi_processor_count
     best = None
     for block, per_lane, chunks in candidates(engine.n_obs_pad, engine.n_src_pad, n_cu):
         cfg = N.TileCfg(block, per_lane, chunks, 0)
-        engine.run(prec, cfg)  # warm-up / allocation
+        _run_and_reduce(engine, prec, cfg)  # warm-up / allocation
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            engine.run(prec, cfg)
+            _run_and_reduce(engine, prec, cfg)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         if verbose:
