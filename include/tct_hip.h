@@ -49,12 +49,16 @@ enum tct_prec { TCT_F32 = 0, TCT_F64 = 1 };
  *   src_chunks:   the source points are split into this many contiguous
  *                 chunks, one grid row each; partial sums are written per
  *                 chunk and summed in a fixed order by the reduce/scatter
- *                 step, so results are bit-reproducible. */
+ *                 step, so results are bit-reproducible;
+ *   flags:        bit 0 keeps the FP32 kernel's r2 <= eps2 test on every
+ *                 source tile (by default tiles too far from a wave's
+ *                 observation groups to hold such a pair skip it; the
+ *                 results are the same). */
 typedef struct {
     int32_t block;
     int32_t obs_per_lane;
     int32_t src_chunks;
-    int32_t reserved;
+    int32_t flags;
 } tct_tile_cfg;
 
 /* Points per origin group of the FP32 layout (obs and src).  FP32

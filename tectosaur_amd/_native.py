@@ -30,7 +30,7 @@ class TctError(RuntimeError):
 
 class TileCfg(ctypes.Structure):
     _fields_ = [('block', ctypes.c_int32), ('obs_per_lane', ctypes.c_int32),
-                ('src_chunks', ctypes.c_int32), ('reserved', ctypes.c_int32)]
+                ('src_chunks', ctypes.c_int32), ('flags', ctypes.c_int32)]
 
 
 _vp = ctypes.c_void_p

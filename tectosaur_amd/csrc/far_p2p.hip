@@ -29,11 +29,15 @@ namespace tct {
 // FP32 N-body.  grid = (ceil(n_obs_pad / (blockDim*P)), chunks).
 // loc32 [6][n_obs_pad]; obs_org [n_obs_pad/G][3]; rec [n_src_pad][RW];
 // src_org [n_src_pad/G][3]; partial [chunks][NO][n_obs_pad].
+// sep2: a source tile whose origin is farther than sqrt(sep2) from the
+// wave's observation group origins holds no pair with r2 <= eps2, so its
+// pairs skip the test (the same values: the test passes them through).
+// The last tile (it holds the padding records) always keeps the test.
 template <int KID, int P>
 __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
     int n_obs_pad, const float *__restrict__ loc32, const double *__restrict__ obs_org,
     int n_tiles, const float *__restrict__ rec, const double *__restrict__ src_org,
-    KConst<float> k, double *__restrict__ partial) {
+    KConst<float> k, double sep2, double *__restrict__ partial) {
     constexpr int RW = rec_words<KID>();
     constexpr int NO = n_out<KID>();
     const int bd = blockDim.x;
@@ -76,11 +80,28 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
             for (int a = 0; a < n_mterm_arr<KID>(); ++a) am[p][a] = 0.0f;
         }
         const float *r = rec + (size_t)t * G * RW;
-#pragma unroll 4
-        for (int j = 0; j < G; ++j) {
-            const float *q = r + j * RW;
+        bool close = t == n_tiles - 1;
 #pragma unroll
-            for (int p = 0; p < P; ++p) pair_eval<KID, float>(xs[p], m[p], q, k, acc[p], am[p]);
+        for (int p = 0; p < P; ++p) {
+            const double d0 = os0 - oo[p][0], d1 = os1 - oo[p][1], d2 = os2 - oo[p][2];
+            close = close || d0 * d0 + d1 * d1 + d2 * d2 <= sep2;
+        }
+        if (__any(close)) {
+#pragma unroll 4
+            for (int j = 0; j < G; ++j) {
+                const float *q = r + j * RW;
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    pair_eval<KID, float, true>(xs[p], m[p], q, k, acc[p], am[p]);
+            }
+        } else {
+#pragma unroll 4
+            for (int j = 0; j < G; ++j) {
+                const float *q = r + j * RW;
+#pragma unroll
+                for (int p = 0; p < P; ++p)
+                    pair_eval<KID, float, false>(xs[p], m[p], q, k, acc[p], am[p]);
+            }
         }
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -444,9 +465,9 @@ __global__ void scatter_curl_kernel(int64_t n_tris, int nq, const int32_t *__res
 template <int KID, int P>
 static void launch_f32(dim3 grid, int block, hipStream_t st, int n_obs_pad, const float *loc32,
                        const double *obs_org, int n_tiles, const float *rec,
-                       const double *src_org, KConst<float> k, double *partial) {
+                       const double *src_org, KConst<float> k, double sep2, double *partial) {
     hipLaunchKernelGGL((far_p2p_f32_kernel<KID, P>), grid, dim3(block), 0, st, n_obs_pad, loc32,
-                       obs_org, n_tiles, rec, src_org, k, partial);
+                       obs_org, n_tiles, rec, src_org, k, sep2, partial);
 }
 
 template <int KID, int P>
@@ -593,9 +614,21 @@ extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float 
               (unsigned)c.src_chunks);
     hipStream_t st = as_stream(stream);
     const int no = (int)n_obs_pad;
+    // eps2 is (8 u (E_obs + E_src))^2 with u = 2^-24 and E the largest
+    // distance of a point from its group origin, so points of groups whose
+    // origins are d apart are at least d - (E_obs + E_src) apart; beyond
+    // sep (with room for the rounding of the local coordinates) no pair can
+    // be skipped by the test.  eps2 = 0, or bit 0 of cfg.flags, keeps
+    // the test everywhere.
+    double sep2 = INFINITY;
+    if (eps2 > 0.0 && !(c.flags & 1)) {
+        const double e = std::sqrt(eps2);
+        const double sep = e / (8.0 * 0x1p-24) * (1.0 + 1e-3) + 4.0 * e;
+        sep2 = sep * sep;
+    }
 #define TCT_F32_CASE(KID, P)                                                                 \
     launch_f32<KID, P>(grid, c.block, st, no, loc32, obs_org, n_tiles, (const float *)rec,    \
-                       src_org, kernel_consts<float>(KID, params_host, eps2), out_partial)
+                       src_org, kernel_consts<float>(KID, params_host, eps2), sep2, out_partial)
 #define TCT_F64_CASE(KID, P)                                                              \
     launch_f64<KID, P>(grid, c.block, st, no, obs_soa64, n_tiles, (const double *)rec,     \
                        kernel_consts<double>(KID, params_host, 0.0), out_partial)

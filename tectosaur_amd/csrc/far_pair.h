@@ -20,6 +20,10 @@ struct KConst {
     T c0, c1, c2, c3, c4, eps2;
 };
 
+// Without the test, for tiles where no pair can reach r2 <= eps2.
+__device__ __forceinline__ float rsq_plain(float r2) { return __builtin_amdgcn_rsqf(r2); }
+__device__ __forceinline__ double rsq_plain(double r2) { return rsqrt(r2); }
+
 __device__ __forceinline__ float rsq_masked(float r2, float eps2) {
     float v = __builtin_amdgcn_rsqf(r2);
     return r2 > eps2 ? v : 0.0f;
@@ -54,12 +58,12 @@ constexpr int n_mterm_arr() { return n_mterm<KID>() > 0 ? n_mterm<KID>() : 1; }
 //   RA:      q = [y - s -], s = w J in(q);
 //   RH:      q = [y G(9) P(9) -], G = w J CsRH0 S,
 //            P = w J CsRH0 (CsRH1 tr(S) I + CsRH2 S^T - 2 S).
-template <int KID, typename T>
+template <int KID, typename T, bool MASK = true>
 __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *__restrict__ q,
                                           const KConst<T> &k, T *acc, T *am) {
     const T D0 = q[0] - x[0], D1 = q[1] - x[1], D2 = q[2] - x[2];
     const T r2 = D0 * D0 + D1 * D1 + D2 * D2;
-    const T invr = rsq_masked(r2, k.eps2);
+    const T invr = MASK ? rsq_masked(r2, k.eps2) : rsq_plain(r2);
     const T w = invr * invr;
     if constexpr (KID == TCT_U) {
         // U: (3-4nu)/r s + (D.s) D / r^3, strengths scaled by CsU1.

@@ -214,3 +214,26 @@ def test_point_form_refuses_surface_curl_kernels():
     p = np.random.default_rng(0).random((10, 3))
     with pytest.raises(N.TctError):
         farfield_pts_direct('elasticRH3', p, p, p, p, p.reshape(-1), [1.0, 0.25], np.float32)
+
+
+@pytest.mark.parametrize('K', ['elasticU3', 'elasticT3', 'elasticA3', 'elasticH3', 'elasticRT3',
+                               'elasticRA3', 'elasticRH3'])
+def test_tiles_without_the_r2_test_change_nothing(K):
+    # source tiles far from a wave's observation groups skip the r2 <= eps2
+    # test; forcing the test everywhere (flags bit 0) must give the same
+    # bits, for one point set packed twice and for overlapping subsets
+    # packed differently (shared points in unrelated groups)
+    from tectosaur_amd import _native as N
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = cfg1_mesh()
+    n = tris.shape[0]
+    for obs, src in ((np.arange(n), np.arange(n)), (np.arange(n // 2 + 300),
+                                                    np.arange(n // 2 - 300, n))):
+        outs = []
+        for flags in (0, 1):
+            op = TriToTriDirectFarfieldOp(2, K, [3e10, 0.25], pts, tris, np.float32, obs, src,
+                                          tile_cfg=N.TileCfg(128, 1, 4, flags))
+            v = np.random.default_rng(4).normal(size=op.shape[1])
+            outs.append(op.dot(v))
+        assert np.all(np.isfinite(outs[0]))
+        assert np.array_equal(outs[0], outs[1]), (K, obs.shape[0], src.shape[0])
