@@ -135,7 +135,11 @@ int tct_far_gather_curl(int kernel, int prec, int64_t n_pts, int nq, const int32
  * skipped.  TCT_F32: FP32 evaluation and FP32 accumulation within each
  * 64-point source tile, FP64 accumulation across tiles; pairs with
  * r2 <= eps2 are skipped, where eps2 bounds the rounding of one point stored
- * relative to two different group origins (see the packing).  TCT_F64: FP64
+ * relative to two different group origins: (8 * 2^-24 * (E_obs + E_src))^2,
+ * E the largest distance of a point from its packed group origin.  Unless
+ * cfg->flags bit 0 is set, tiles are freed from the test on the assumption
+ * that eps2 is at least that value (a smaller nonzero eps2 needs bit 0;
+ * eps2 = 0 tests only r2 == 0, on every tile).  TCT_F64: FP64
  * throughout, absolute coordinates, exact r2 == 0 test (eps2 ignored).
  * Kernels U, T, A, H (3 sums per point) and the surface-curl forms RT (3),
  * RA and RH (9 moments per point, applied to the observation curls by

@@ -25,6 +25,31 @@
 #include "far_pair.h"
 
 namespace tct {
+// Words of a source record that the pair body of KID reads.
+template <int KID>
+constexpr int used_words() {
+    return KID == TCT_RH ? 21 : KID == TCT_RT ? 12 : KID == TCT_H ? 11 : KID == TCT_T ? 10 : 9;
+}
+// U consecutive records (the first NW of their RW words) into registers.
+template <int U, int NW, int RW>
+__device__ __forceinline__ void load_recs(float (&q)[U][NW], const float *__restrict__ r) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int w = 0; w < NW; ++w) q[u][w] = r[u * RW + w];
+}
+
+// An empty statement that reads the last loaded word (so the scalar loads
+// before it are waited for here), orders memory accesses around it, and
+// passes the lane's coordinates through, so the pairs that use them are not
+// moved above it.
+template <int P>
+__device__ __forceinline__ void fence_recs(float (&xs)[P][3], float last) {
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+        asm volatile("" : "+v"(xs[p][0]), "+v"(xs[p][1]), "+v"(xs[p][2]) : "s"(last) : "memory");
+}
+
 // ---------------------------------------------------------------------------
 // FP32 N-body.  grid = (ceil(n_obs_pad / (blockDim*P)), chunks).
 // loc32 [6][n_obs_pad]; obs_org [n_obs_pad/G][3]; rec [n_src_pad][RW];
@@ -95,12 +120,30 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
                     pair_eval<KID, float, true>(xs[p], m[p], q, k, acc[p], am[p]);
             }
         } else {
-#pragma unroll 4
-            for (int j = 0; j < G; ++j) {
-                const float *q = r + j * RW;
+            // Records of the next U source points are loaded while the
+            // current U compute: scalar loads return out of order, so a wait
+            // for one waits for all; fence_recs places that wait before the
+            // next loads are issued and the pairs that need it after it.
+            constexpr int NW = used_words<KID>();
+            constexpr int U = NW > 12 ? 1 : 2;
+            float qa[U][NW], qb[U][NW];
+            load_recs<U, NW, RW>(qa, r);
+#pragma unroll 1
+            for (int j = 0; j < G; j += 2 * U) {
+                fence_recs<P>(xs, qa[U - 1][NW - 1]);
+                load_recs<U, NW, RW>(qb, r + (j + U) * RW);
 #pragma unroll
-                for (int p = 0; p < P; ++p)
-                    pair_eval<KID, float, false>(xs[p], m[p], q, k, acc[p], am[p]);
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int p = 0; p < P; ++p)
+                        pair_eval<KID, float, false>(xs[p], m[p], qa[u], k, acc[p], am[p]);
+                fence_recs<P>(xs, qb[U - 1][NW - 1]);
+                load_recs<U, NW, RW>(qa, r + min(j + 2 * U, G - U) * RW);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int p = 0; p < P; ++p)
+                        pair_eval<KID, float, false>(xs[p], m[p], qb[u], k, acc[p], am[p]);
             }
         }
 #pragma unroll
