@@ -202,6 +202,8 @@ extern "C" int tct_tick_gather(int n_streams, void *const *streams, void *const 
     }
     TCT_CHECK_ARG(total <= staging_doubles, "tick_gather: %lld values exceed the staging buffer",
                   (long long)total);
+    TCT_CHECK_ARG(total < (1LL << 31), "tick_gather: %lld values (offsets are 32-bit)",
+                  (long long)total);
     hipStream_t st = as_stream(main_stream);
     hipError_t e = hipSuccess;
     for (int i = 0; i < n_streams && e == hipSuccess; ++i) {
