@@ -59,30 +59,54 @@ __global__ __launch_bounds__(KB) void combine_kernel(int m, int64_t n, int64_t l
     out[t] = (beta == 0.0 ? 0.0 : beta * out[t]) + alpha * s;
 }
 
-// The reduce and the update in one launch: every block first sums the
-// per-block partials (fixed order, the same in every block) into LDS:
-// c[i] = sum_b partial[b][i]; block 0 also writes c (+ add) to hout.  Then
-// w[t] -= sum_i V[i][t] c[i] over the block's points (grid-stride).
-__global__ __launch_bounds__(KB) void reduce_combine_kernel(int m, int64_t n, int64_t ld,
-                                                            const double *__restrict__ V,
-                                                            const double *__restrict__ partial,
-                                                            int nb,
-                                                            const double *__restrict__ add,
-                                                            double *__restrict__ hout,
-                                                            double *__restrict__ w) {
+// The reduce, the update and the next sums in one launch, on the partition
+// of multi_dot_kernel (gridDim.x == nb blocks of contiguous points): every
+// block sums partial_in[b][0..m) in fixed order into LDS c, block 0 writes
+// c (+ add) to hout; w[t] -= sum_i V[i][t] c[i] on the block's points; then
+// partial_out[b][i] = sum over those points of V[i][t] w[t] (norm_only: the
+// one column sum w[t]^2).  Each w[t] is updated by one sum in a fixed order
+// and the sums follow multi_dot_kernel's per-thread order, so the values do
+// not depend on the launch shape and match an update launch followed by
+// multi_dot_kernel bit for bit; each thread reads back only the w[t] it
+// wrote.  partial_in and partial_out must not overlap.
+__global__ __launch_bounds__(KB) void reduce_update_dot_kernel(
+    int m, int64_t n, int64_t ld, const double *__restrict__ V,
+    const double *__restrict__ partial_in, const double *__restrict__ add,
+    double *__restrict__ hout, double *__restrict__ w, int norm_only,
+    double *__restrict__ partial_out) {
     extern __shared__ double c[];
+    __shared__ double red[KB / 64];
+    const int nb = gridDim.x;
     for (int i = threadIdx.x; i < m; i += KB) {
         double s = 0.0;
-        for (int b = 0; b < nb; ++b) s += partial[(size_t)b * m + i];
+        for (int b = 0; b < nb; ++b) s += partial_in[(size_t)b * m + i];
         c[i] = s;
         if (blockIdx.x == 0 && hout) hout[i] = add ? s + add[i] : s;
     }
     __syncthreads();
-    for (int64_t t = (int64_t)blockIdx.x * KB + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * KB) {
+    const int64_t per = (n + nb - 1) / nb;
+    const int64_t t0 = (int64_t)blockIdx.x * per;
+    const int64_t t1 = t0 + per < n ? t0 + per : n;
+    for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) {
         double s = 0.0;
         for (int i = 0; i < m; ++i) s += V[(size_t)i * ld + t] * c[i];
         w[t] = w[t] - s;
+    }
+    const int cols = norm_only ? 1 : m;
+    for (int i = 0; i < cols; ++i) {
+        const double *vi = norm_only ? w : V + (size_t)i * ld;
+        double s = 0.0;
+        for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) s += vi[t] * w[t];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r = 0.0;
+            for (int k = 0; k < KB / 64; ++k) r += red[k];
+            partial_out[(size_t)blockIdx.x * cols + i] = r;
+        }
+        __syncthreads();
     }
 }
 
@@ -140,8 +164,9 @@ using namespace tct;
 
 extern "C" int tct_krylov_scratch(int m, int64_t n, int64_t *n_doubles) {
     TCT_CHECK_ARG(m >= 1 && n >= 1 && n_doubles, "krylov_scratch: bad arguments");
-    // partials for m + 1 columns, plus h1 [m]
-    *n_doubles = (int64_t)n_blocks(n) * (m + 1) + m;
+    // two partial arrays of m columns per block (one read while the other
+    // is written), plus h1 [m]
+    *n_doubles = 2 * (int64_t)n_blocks(n) * m + m;
     return TCT_OK;
 }
 
@@ -152,27 +177,27 @@ extern "C" int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w
                   (long long)n, (long long)ld);
     TCT_CHECK_ARG(V && w && h && scratch, "cgs2: null pointer");
     const int nb = n_blocks(n);
-    TCT_CHECK_ARG(scratch_doubles >= (int64_t)nb * (m + 1) + m, "cgs2: scratch too small");
+    TCT_CHECK_ARG(scratch_doubles >= 2 * (int64_t)nb * m + m, "cgs2: scratch too small");
     hipStream_t st = as_stream(stream);
-    double *partial = scratch;
-    double *h1 = scratch + (size_t)nb * (m + 1);
-    // update launches: enough blocks to cover the points, at most 1024
+    double *p1 = scratch;                     // [nb][m], later the norm partials
+    double *p2 = scratch + (size_t)nb * m;    // [nb][m]
+    double *h1 = scratch + 2 * (size_t)nb * m;
+    // normalisation launch: enough blocks to cover the points, at most 1024
     const int64_t gpl = ceil_div(n, KB);
     const unsigned gp = (unsigned)(gpl < 1024 ? gpl : 1024);
     const size_t lds = sizeof(double) * (size_t)m;
     TCT_CHECK_ARG(lds <= 60 * 1024, "cgs2: basis size %d too large for LDS", m);
-    // pass 1: h1 = V w; w -= V^T h1
-    hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, m, n, ld, V, w, 0, partial);
-    hipLaunchKernelGGL(reduce_combine_kernel, dim3(gp), dim3(KB), lds, st, m, n, ld, V, partial,
-                       nb, (const double *)nullptr, h1, w);
-    // pass 2: h2 = V w; w -= V^T h2; h = h2 + h1
-    hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, m, n, ld, V, w, 0, partial);
-    hipLaunchKernelGGL(reduce_combine_kernel, dim3(gp), dim3(KB), lds, st, m, n, ld, V, partial,
-                       nb, (const double *)h1, h, w);
+    // p1 = partial sums of V w
+    hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, m, n, ld, V, w, 0, p1);
+    // h1 = sum p1; w -= V^T h1; p2 = partial sums of V w
+    hipLaunchKernelGGL(reduce_update_dot_kernel, dim3(nb), dim3(KB), lds, st, m, n, ld, V,
+                       (const double *)p1, (const double *)nullptr, h1, w, 0, p2);
+    // h2 = sum p2, h = h1 + h2; w -= V^T h2; p1 = partial sums of w^2
+    hipLaunchKernelGGL(reduce_update_dot_kernel, dim3(nb), dim3(KB), lds, st, m, n, ld, V,
+                       (const double *)p2, (const double *)h1, h, w, 1, p1);
     // h[m] = |w|, and the next basis vector
-    hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, 0, n, ld, V, w, 1, partial);
     hipLaunchKernelGGL(norm_normalize_kernel, dim3(vnext ? gp : 1), dim3(KB), 0, st, n, w,
-                       partial, nb, h + m, vnext, vnext2);
+                       p1, nb, h + m, vnext, vnext2);
     TCT_CHECK_LAUNCH("cgs2");
     return TCT_OK;
 }
