@@ -174,6 +174,22 @@ __device__ __forceinline__ double final_value(const double *acc, const Tri &ot, 
     }
 }
 
+// Rule point iq: (obs xi, obs eta, src xi, src eta, weight).
+__device__ __forceinline__ void load_point(double (&q)[5], const double *__restrict__ qx,
+                                           const double *__restrict__ qw, int iq) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = qx[4 * (size_t)iq + k];
+    q[4] = qw[iq];
+}
+// An empty statement that reads the last loaded word of the point (so the
+// scalar loads are waited for here), keeps memory accesses on their side of
+// it, and passes one accumulator and the pair's base offset b through: the
+// previous point's update of that accumulator comes before it and the next
+// point's arithmetic, which starts from b, after it.
+__device__ __forceinline__ void fence_point(double last, double (&b)[3], double &dep) {
+    asm volatile("" : "+v"(b[0]), "+v"(b[1]), "+v"(b[2]), "+v"(dep) : "s"(last) : "memory");
+}
+
 template <int KID, int GL, bool CHECK0>
 __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__restrict__ qx,
                                                     const double *__restrict__ qw,
@@ -205,9 +221,33 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
     double acc[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
-    if (live)
+    if constexpr (GL == 1 && !CHECK0) {
+        // one pair per lane: every lane walks the same rule, whose points are
+        // wave-uniform scalar loads.  Those return out of order, so a wait
+        // for one waits for all: the next point is loaded while the current
+        // one computes, and fence_point puts the one wait per point after
+        // the previous point's arithmetic and before the next load is issued
+        // and the arithmetic that uses the point.
+        if (live && nq > 0) {
+            double qa[5], qb[5];
+            load_point(qa, qx, qw, 0);
+            for (int iq = 0; iq + 1 < nq; iq += 2) {
+                fence_point(qa[4], g.b, acc[NA - 1]);
+                load_point(qb, qx, qw, iq + 1);
+                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc);
+                fence_point(qb[4], g.b, acc[NA - 1]);
+                load_point(qa, qx, qw, iq + 2 < nq ? iq + 2 : nq - 1);
+                accumulate_point<KID, CHECK0>(ot, st, g, qb, qb[4], c, acc);
+            }
+            if (nq & 1) {   // the last point, loaded by the final clamp
+                fence_point(qa[4], g.b, acc[NA - 1]);
+                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc);
+            }
+        }
+    } else if (live) {
         for (int iq = sub; iq < nq; iq += GL)
             accumulate_point<KID, CHECK0>(ot, st, g, qx + 4 * (size_t)iq, qw[iq], c, acc);
+    }
     if constexpr (GL > 1) {
 #pragma unroll
         for (int k = 0; k < NA; ++k) {
