@@ -58,6 +58,9 @@ constexpr int n_mterm_arr() { return n_mterm<KID>() > 0 ? n_mterm<KID>() : 1; }
 //   RA:      q = [y - s -], s = w J in(q);
 //   RH:      q = [y G(9) P(9) -], G = w J CsRH0 S,
 //            P = w J CsRH0 (CsRH1 tr(S) I + CsRH2 S^T - 2 S).
+// The FP32 N-body's pipelined loop copies only the leading words each body
+// reads (used_words in far_p2p.hip: U/A/RA 9, T 10, H 11, RT 12, RH 21); a
+// body that reads a later word must raise that count.
 template <int KID, typename T, bool MASK = true>
 __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *__restrict__ q,
                                           const KConst<T> &k, T *acc, T *am) {
