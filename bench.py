@@ -39,7 +39,7 @@ RH_NEAR_FLOPS_PER_QP = 65
 # WRITE_SIZE in separate runs, scaled x2 and x1 by a 1 GiB copy measured in
 # the same runs): profiles/far_p2p_H_two_surface_442k_pmc_traffic.md.
 TRAFFIC_FAR_P2P_H_F32 = 1.752e9
-TRAFFIC_BSR_ICOSPHERE_L5 = 1.330e9
+TRAFFIC_BSR_ICOSPHERE_L5 = 1.326e9
 TRAFFIC_SOURCE = 'profiles/far_p2p_H_two_surface_442k_pmc_traffic.md'
 
 

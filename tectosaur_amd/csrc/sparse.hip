@@ -26,19 +26,32 @@ __global__ __launch_bounds__(256) void bsr9_kernel(int n_block_rows,
                                                    double *__restrict__ y, int accumulate) {
     constexpr int NB = 7;
     __shared__ double part[4][NB * 9];
+    __shared__ int scol[4][64];
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + wave;
     const bool live = row < n_block_rows;
     const int b = lane / 9, i = lane % 9;
     double s = 0.0;
-    if (live && lane < NB * 9) {
+    if (live) {
         const int k0 = indptr[row], k1 = indptr[row + 1];
-        for (int k = k0 + b; k < k1; k += NB) {
-            const TD *A = data + (size_t)k * 81 + i * 9;
-            const double *xc = x + (size_t)indices[k] * 9;
+        // the column indices of the next 63 blocks (9 steps of NB) come in
+        // one coalesced load through the wave's LDS slot, so no block waits
+        // on its own index load before its data and x loads can issue
+        for (int kb = k0; kb < k1; kb += 9 * NB) {
+            scol[wave][lane] = kb + lane < k1 ? indices[kb + lane] : 0;
+            __builtin_amdgcn_wave_barrier();
+            const int ke = kb + 9 * NB < k1 ? kb + 9 * NB : k1;
+            if (lane < NB * 9) {
+#pragma unroll 2
+                for (int k = kb + b; k < ke; k += NB) {
+                    const TD *A = data + (size_t)k * 81 + i * 9;
+                    const double *xc = x + (size_t)scol[wave][k - kb] * 9;
 #pragma unroll
-            for (int j = 0; j < 9; ++j) s += (double)A[j] * xc[j];
+                    for (int j = 0; j < 9; ++j) s += (double)A[j] * xc[j];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
     }
     if (lane < NB * 9) part[wave][lane] = s;
