@@ -53,7 +53,7 @@ enum tct_prec { TCT_F32 = 0, TCT_F64 = 1 };
  *   flags:        bit 0 keeps the FP32 kernel's r2 <= eps2 test on every
  *                 source tile (by default tiles too far from a wave's
  *                 observation groups to hold such a pair skip it; the
- *                 results are the same). */
+ *                 results are the same); other bits must be 0. */
 typedef struct {
     int32_t block;
     int32_t obs_per_lane;

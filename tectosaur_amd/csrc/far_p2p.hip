@@ -648,6 +648,7 @@ extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float 
     tct_tile_cfg c{256, 2, 1, 0};
     if (cfg) c = *cfg;
     TCT_CHECK_ARG(c.block == 64 || c.block == 128 || c.block == 256, "far_p2p: block %d", c.block);
+    TCT_CHECK_ARG((c.flags & ~1) == 0, "far_p2p: unknown flags 0x%x", (unsigned)c.flags);
     TCT_CHECK_ARG(c.obs_per_lane == 1 || c.obs_per_lane == 2 || c.obs_per_lane == 4,
                   "far_p2p: obs_per_lane %d", c.obs_per_lane);
     const int n_tiles = (int)(n_src_pad / G);
