@@ -112,12 +112,27 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
             close = close || d0 * d0 + d1 * d1 + d2 * d2 <= sep2;
         }
         if (__any(close)) {
-#pragma unroll 4
-            for (int j = 0; j < G; ++j) {
-                const float *q = r + j * RW;
+            // the same pipelined loop with the r2 test (see below)
+            constexpr int NW = used_words<KID>();
+            constexpr int U = NW > 12 ? 1 : 2;
+            float qa[U][NW], qb[U][NW];
+            load_recs<U, NW, RW>(qa, r);
+#pragma unroll 1
+            for (int j = 0; j < G; j += 2 * U) {
+                fence_recs<P>(xs, qa[U - 1][NW - 1]);
+                load_recs<U, NW, RW>(qb, r + (j + U) * RW);
 #pragma unroll
-                for (int p = 0; p < P; ++p)
-                    pair_eval<KID, float, true>(xs[p], m[p], q, k, acc[p], am[p]);
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int p = 0; p < P; ++p)
+                        pair_eval<KID, float, true>(xs[p], m[p], qa[u], k, acc[p], am[p]);
+                fence_recs<P>(xs, qb[U - 1][NW - 1]);
+                load_recs<U, NW, RW>(qa, r + min(j + 2 * U, G - U) * RW);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int p = 0; p < P; ++p)
+                        pair_eval<KID, float, true>(xs[p], m[p], qb[u], k, acc[p], am[p]);
             }
         } else {
             // Records of the next U source points are loaded while the
