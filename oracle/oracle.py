@@ -19,6 +19,12 @@ Pinning (what fixes each function independently of this restatement):
       test_farfield_two_tris.npy (H, nq=3); test_gpu_vert_adjacent.npy
       (H, vertex-adjacent, nq=3); refinement additivity (a parent coincident
       integral equals the sum of its 16 child-pair integrals).
+  pairs_integrate (edge-adjacent, flipped source)  the six numberings of
+      one source triangle give the same block per source vertex up to the
+      normal's sign; the reference's de-rotation of a flipped source
+      (assemble.cl:85-89) departs from that for nonzero clicks, and the
+      oracle reproduces exactly the computed departure
+      (tests/test_oracle.py::test_edge_adjacent_numbering_invariance).
   pairs_integrate (RT/RA/RH)      refinement additivity and convergence in n;
       parity unpinned against the reference's numbers (no fixture exists and
       the reference cannot run here).
@@ -42,9 +48,14 @@ Pinning (what fixes each function independently of this restatement):
       subdivision of the triangle (brute force) for points above, beside
       and near the edges of a triangle; the rigid-body identity at points
       1e-3 triangle sizes from a closed sphere.
-  interior_corners                parity unpinned: the reference's corner
-      integrals have no fixture and no closed form fixes their finite-part
-      convention; checked only against the HIP restatement.
+  corner_radial / interior_corners  the radial integrals I1, I2, I3 equal an
+      independent sympy derivation of the eps-regularised finite parts
+      (tests/golden/make_corner_golden.py, cross-checked there by mpmath
+      quadrature) to 1e-12 at 24 random (a, rho_max, n); I4 is zero as the
+      reference's integer division makes it (the derived I4 is not).  The
+      whole corner block equals an adaptive-quadrature composition of those
+      derived integrals with the P1 basis, for each vertex of the source
+      triangle (tests/test_oracle.py::test_interior_corner_blocks).
   interior_op_dot                 composed from the pinned pieces above;
       the test_interior.npy fixture pins it whole when nq_far == nq_near.
 """
@@ -94,6 +105,7 @@ def lib():
                                              _d]
         L.or_interior_corners.argtypes = [ctypes.c_int64, _d, _d, _d, _d, _i32, ctypes.c_int64,
                                           _i64, _d, _d]
+        L.or_corner_radial.argtypes = [_d, ctypes.c_double, _d, _d]
         _LIB = L
     return _LIB
 
@@ -261,28 +273,62 @@ def bcoo_matvec(rows, cols, data, x, shape):
 
 
 def nearfield_blocks(pts, tris, nq_co, nq_ea, nq_va, nq_far, nq_near, near_threshold,
-                     K_near, K_far, params, rules):
-    """The nearfield assembly of tectosaur/nearfield/nearfield_op.py:95-141
-    for obs_subset == src_subset == all triangles, computed with the oracle
-    pair integrator.  ``rules`` supplies the quadrature rules (a dict with
-    'co', 'ea', 'va', 'near', 'far' -> (qx, qw)) so this module does not
-    depend on the product package.  Returns a dict class -> (pairs, corrected
-    blocks [n,9,9], uncorrected blocks [n,9,9])."""
+                     K_near, K_far, params, rules, obs_subset=None, src_subset=None):
+    """The nearfield assembly of tectosaur/nearfield/nearfield_op.py:95-141,
+    computed with the oracle pair integrator, for any obs/src triangle
+    subsets (default: all triangles).  ``rules`` supplies the quadrature
+    rules (a dict with 'co', 'ea', 'va', 'near', 'far' -> (qx, qw)) so this
+    module does not depend on the product package.  Returns a dict class ->
+    (pairs in triangle space, pairs in DOF space (positions within the
+    subsets), corrected blocks [n,9,9], uncorrected blocks [n,9,9])."""
     n = tris.shape[0]
-    co = np.stack([np.arange(n), np.arange(n)], axis=1)
-    close = near_pairs(pts, tris, pts, tris, near_threshold)
-    near, va4, ea6 = split_adjacent_close(close, tris, tris)
-    va = np.concatenate([va4, np.zeros((va4.shape[0], 1), dtype=np.int64)], axis=1)
-    ea = resolve_ea_rotation(tris, ea6)
+    obs_subset = np.arange(n) if obs_subset is None else np.asarray(obs_subset, dtype=np.int64)
+    src_subset = np.arange(n) if src_subset is None else np.asarray(src_subset, dtype=np.int64)
+
+    def dof_of(tri_pairs):
+        # nearfield_op.py:24-38: positions by search, one pair at a time
+        out = [[np.where(obs_subset == a)[0][0], np.where(src_subset == b)[0][0]]
+               for a, b in tri_pairs]
+        return np.array(out, dtype=np.int64).reshape((-1, 2))
+
+    def tri_of(dof_pairs):
+        # nearfield_op.py:40-42
+        dp = np.asarray(dof_pairs, dtype=np.int64)
+        t = np.stack([obs_subset[dp[:, 0]], src_subset[dp[:, 1]]], axis=1)
+        return np.concatenate([t, dp[:, 2:]], axis=1)
+
+    co_tris = np.intersect1d(obs_subset, src_subset)
+    co = np.stack([co_tris, co_tris], axis=1)
+    co_dofs = dof_of(co)
+    close = near_pairs(pts, tris[obs_subset], pts, tris[src_subset], near_threshold)
+    near_d, va_d, ea_d = split_adjacent_close(close, tris[obs_subset], tris[src_subset])
+    near = tri_of(near_d)
+    va = tri_of(va_d)
+    va = np.concatenate([va, np.zeros((va.shape[0], 1), dtype=np.int64)], axis=1)
+    ea = resolve_ea_rotation(tris, tri_of(ea_d))
     out = {}
-    for name, prs, q, check0 in (('co', co, rules['co'], True), ('ea', ea, rules['ea'], True),
-                                 ('va', va, rules['va'], True),
-                                 ('near', near, rules['near'], False)):
+    for name, prs, dofs, q, check0 in (
+            ('co', co, co_dofs, rules['co'], True),
+            ('ea', ea, ea_d[:, :2], rules['ea'], True),
+            ('va', va, va_d[:, :2], rules['va'], True),
+            ('near', near, near_d[:, :2], rules['near'], False)):
         acc = pairs_integrate(K_near, q[0], q[1], pts, tris, prs, check0, params)
         corr = pairs_integrate(K_far, rules['far'][0], rules['far'][1], pts, tris,
                                prs[:, :2], name == 'co', params)
-        out[name] = (prs, (acc - corr).reshape((-1, 9, 9)), acc.reshape((-1, 9, 9)))
+        out[name] = (prs, dofs, (acc - corr).reshape((-1, 9, 9)), acc.reshape((-1, 9, 9)))
     return out
+
+
+def nearfield_dense(blocks, n_obs, n_src, corrected=True):
+    """Dense [n_obs*9, n_src*9] matrix of ``nearfield_blocks``' output
+    (the sum of the reference's four BCOO matrices, nearfield_op.py:135-150)."""
+    M = np.zeros((n_obs * 9, n_src * 9))
+    for prs, dofs, corr, unc in blocks.values():
+        B = corr if corrected else unc
+        for k in range(dofs.shape[0]):
+            i, j = dofs[k]
+            M[9 * i:9 * i + 9, 9 * j:9 * j + 9] += B[k]
+    return M
 
 
 def interior_pairs(K, qx, qw, obs_pts, obs_ns, pts, tris, pairs, check0, finite_part, params):
@@ -318,6 +364,16 @@ def interior_pairs_sinh(K, t_rule, r_rule, obs_pts, obs_ns, pts, tris, pairs, pa
                                      _p(rq), _p(rw), _p(obs_pts), _p(obs_ns), _p(pts),
                                      _p(tris, _i32), n, _p(pairs, _i64), _p(params), _p(out))
     return out.reshape((n, 3, 3, 3))
+
+
+def corner_radial(a, rho_max, n):
+    """The radial integrals of the interior-corner rule at one angle
+    (tectosaur/interior_corners.cl:79-83): (I1, I2, I3[3, 3], I4[3, 3]) with
+    I4 the reference's value (zero: C integer division)."""
+    a, n = _f64(a), _f64(n)
+    out = np.zeros(20)
+    lib().or_corner_radial(_p(a), float(rho_max), _p(n), _p(out))
+    return out[0], out[1], out[2:11].reshape(3, 3), out[11:20].reshape(3, 3)
 
 
 def interior_corners(qt, qw, obs_ns, pts, tris, pairs, params):

@@ -760,6 +760,27 @@ void or_interior_pairs(int kid, int64_t n_q, const double *qx, const double *qw,
 /* makes both factors zero, so I4 = 0 there and here.                   */
 /* pairs[n][3] = (obs point, src triangle, vertex index); out[n*27].     */
 /* ------------------------------------------------------------------ */
+
+/* The radial integrals of interior_corners.cl:79-83 for one angle: a is */
+/* the in-plane direction v1 cos t + v2 sin t, rhomax = 1/(cos t+sin t), */
+/* n the source normal.  out = [I1, I2, I3 (3x3 row-major), I4 (3x3)].   */
+/* I4 is the reference's value, zero (its `1/3*(...)` factors are C     */
+/* integer divisions); n enters only through that zeroed I4.            */
+void or_corner_radial(const double *a, double rhomax, const double *n, double *out)
+{
+    (void)n;
+    double A2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    double A = sqrt(A2);
+    double A3 = A2 * A, A5 = A3 * A2;
+    out[0] = -1.0 / (A3 * rhomax);
+    out[1] = (log(2.0) + log(A) + log(rhomax) - 1.0) / A3;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            out[2 + 3 * i + j] = -a[i] * a[j] / (A5 * rhomax);
+            out[11 + 3 * i + j] = 0.0;
+        }
+}
+
 void or_interior_corners(int64_t n_q, const double *qt, const double *qw,
                          const double *obs_ns, const double *src_pts,
                          const int32_t *src_tris, int64_t n_pairs, const int64_t *pairs,
@@ -789,21 +810,18 @@ void or_interior_corners(int64_t n_q, const double *qt, const double *qw,
             double t = qt[iq], w = qw[iq];
             double ct = cos(t), s = sin(t);
             double rhomax = 1.0 / (ct + s);
-            double a[3];
+            double a[3], I[20];
             for (int d = 0; d < 3; d++)
                 a[d] = v1[d] * ct + v2[d] * s;
-            double A2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-            double A = sqrt(A2);
-            double A3 = A2 * A, A5 = A3 * A2;
-            double I1 = -1.0 / (A3 * rhomax);
-            double I2 = (log(2.0) + log(A) + log(rhomax) - 1.0) / A3;
+            or_corner_radial(a, rhomax, st.n, I);
+            double I1 = I[0], I2 = I[1];
             double T0[3] = {-I2 * ct - I2 * s + I1, I2 * ct, I2 * s};
             for (int dob = 0; dob < 3; dob++)
                 for (int b = 0; b < 3; b++)
                     for (int ds = 0; ds < 3; ds++) {
-                        /* T1basis[0] = I3, T1basis[1] = T1basis[2] = 0 (I4 = 0) */
-                        double I3 = -a[dob] * a[ds] / (A5 * rhomax);
-                        double K = b == 0 ? CsH0 * CsH3 * I3 * mn : 0.0;
+                        double I3 = I[2 + 3 * dob + ds], I4 = I[11 + 3 * dob + ds];
+                        double T1[3] = {-I4 * ct - I4 * s + I3, I4 * ct, I4 * s};
+                        double K = CsH0 * CsH3 * T1[b] * mn;
                         if (dob == ds)
                             K += CsH0 * CsH1 * mn * T0[b];
                         res[dob * 9 + b * 3 + ds] += w * K;

@@ -227,3 +227,187 @@ def test_regularized_kernels_equal_singular_forms_on_closed_surfaces(oracle):
     for k in range(3):
         assert errs[6][k] < 2e-9, errs
         assert errs[6][k] < 1e-3 * errs[3][k], errs
+
+
+def test_corner_radial_integrals_match_derivation(oracle):
+    """I1, I2, I3 of the interior-corner rule equal the finite parts derived
+    by tests/golden/make_corner_golden.py (own sympy, cross-checked there by
+    mpmath quadrature) at 24 random (a, rho_max, n); I4 is zero, as
+    tectosaur/interior_corners.cl:83 makes it by C integer division, while
+    the derived I4 is not."""
+    g = np.load(os.path.join(GOLD, 'corner_radial_golden.npz'))
+    assert g['a'].shape[0] >= 16
+    for c in range(g['a'].shape[0]):
+        I1, I2, I3, I4 = oracle.corner_radial(g['a'][c], g['rho_max'][c], g['n'][c])
+        assert abs(I1 - g['I1'][c]) <= 1e-12 * abs(g['I1'][c])
+        assert abs(I2 - g['I2'][c]) <= 1e-12 * max(abs(g['I2'][c]), abs(g['I1'][c]))
+        np.testing.assert_allclose(I3, g['I3'][c], rtol=0, atol=1e-12 * np.abs(g['I3'][c]).max())
+        assert np.all(I4 == 0.0)
+        assert np.abs(g['I4_true'][c]).max() > 0.1 * np.abs(g['I3'][c]).max()
+
+
+def _corner_block_by_extrapolation(tri_pts, k, m, params, with_i4=True):
+    """The mn-part of H (CsH0 mn (CsH3 sep sep^T / r^5 + CsH1 I / r^3), the
+    terms tectosaur/interior_corners.cl:120-146 keeps) integrated against
+    each vertex's P1 basis over the triangle with the observation point
+    lifted by eps along the triangle normal off vertex k, for a ladder of
+    eps; the finite part is the constant term of a least-squares fit in
+    (1/eps, log eps, 1, eps, eps^2, eps^2 log eps).  Everything here is
+    physical-space geometry (barycentric coordinates by a linear solve) and
+    adaptive quadrature; no closed form of the radial integrals is used.
+    Returns [3 (d_obs), 3 (basis of global vertex), 3 (d_src)]."""
+    from scipy.integrate import quad_vec
+    G, nu = params
+    CsH0 = G / (4 * np.pi * (1 - nu))
+    CsH1, CsH3 = 1 - 2 * nu, 3 * nu
+    V = np.asarray(tri_pts, dtype=np.float64)
+    nvec = np.cross(V[1] - V[0], V[2] - V[0])
+    jac = np.linalg.norm(nvec)
+    nvec = nvec / jac
+    mn = float(m @ nvec)
+    v0 = V[k]
+    e1 = V[(k + 1) % 3] - v0
+    e2 = V[(k + 2) % 3] - v0
+    # barycentric coordinates of v0 + xi e1 + eta e2 w.r.t. the global vertices
+    Bm = np.linalg.inv(np.vstack([V.T, np.ones(3)])[[0, 1, 3]] if abs(nvec[2]) > 0.3 else
+                       (np.vstack([V.T, np.ones(3)])[[1, 2, 3]] if abs(nvec[0]) > 0.3 else
+                        np.vstack([V.T, np.ones(3)])[[0, 2, 3]]))
+    sel = [0, 1] if abs(nvec[2]) > 0.3 else ([1, 2] if abs(nvec[0]) > 0.3 else [0, 2])
+    tq, tw = np.polynomial.legendre.leggauss(16)
+    tq = (tq + 1) * np.pi / 4
+    tw = tw * np.pi / 4
+    eps_list = 4e-3 * 2.0 ** -np.arange(7)
+    F = []
+    for eps in eps_list:
+        x = v0 + eps * nvec
+        tot = np.zeros(27)
+        for t, w in zip(tq, tw):
+            ct, st = np.cos(t), np.sin(t)
+            rmax = 1.0 / (ct + st)
+            a = e1 * ct + e2 * st
+            A = np.linalg.norm(a)
+
+            def f(rho):
+                y = v0 + rho * a
+                phi = Bm @ np.concatenate([y[sel], [1.0]])
+                d = y - x
+                r2 = d @ d
+                r = np.sqrt(r2)
+                Kt = CsH0 * mn * (CsH3 * np.outer(d, d) / (r2 * r2 * r) +
+                                  CsH1 * np.eye(3) / (r2 * r))
+                # [d_obs, b, d_src]
+                return (rho * Kt[:, None, :] * phi[None, :, None]).reshape(-1)
+            brk = [eps / A * s for s in (1.0, 10.0, 100.0) if eps / A * s < rmax]
+            val, _ = quad_vec(f, 0.0, rmax, points=brk, epsabs=0.0, epsrel=1e-14, limit=400)
+            tot += w * val
+        F.append(jac * tot)
+    F = np.array(F)
+    e = eps_list
+    Mfit = np.stack([1 / e, np.log(e), np.ones_like(e), e, e * e], axis=1)
+    coef, *_ = np.linalg.lstsq(Mfit, F, rcond=None)
+    return coef[2].reshape(3, 3, 3)
+
+
+def test_interior_corner_blocks(oracle):
+    """The oracle's interior-corner blocks against an independent
+    finite-part evaluation (above), for the observation point at each
+    vertex of a tilted source triangle.  The basis-summed block does not
+    involve I4 (the P1 basis sums to one) and must match directly; per
+    basis, the oracle plus the derived I4's contribution must match."""
+    from tectosaur_amd.quadrature import gaussxw, map_to
+    params = [3e10, 0.25]
+    pts = np.array([[0.1, -0.2, 0.05], [1.2, 0.1, 0.3], [0.3, 0.9, -0.2]])
+    tris = np.array([[0, 1, 2]], dtype=np.int32)
+    rng = np.random.default_rng(7)
+    m = rng.normal(size=3)
+    m /= np.linalg.norm(m)
+    obs_ns = np.tile(m, (3, 1))
+    qt, qw = map_to(gaussxw(40), [0.0, np.pi / 2])
+    pairs = np.array([[k, 0, k] for k in range(3)])
+    got = oracle.interior_corners(qt, qw, obs_ns, pts, tris, pairs, params)
+    g = np.load(os.path.join(GOLD, 'corner_radial_golden.npz'))
+    forms = dict((k, v) for k, v in g['forms'])
+    import sympy as sp
+    As, Rs = sp.symbols('A R', positive=True)
+    fJ45 = sp.lambdify((As, Rs), sp.sympify(forms['J45'], locals={'A': As, 'R': Rs}), 'numpy')
+    fe35 = sp.lambdify((As, Rs), sp.sympify(forms['eJ35'], locals={'A': As, 'R': Rs}), 'numpy')
+    fe25 = sp.lambdify((As, Rs), sp.sympify(forms['e2J25'], locals={'A': As, 'R': Rs}), 'numpy')
+    G, nu = params
+    CsH0, CsH3 = G / (4 * np.pi * (1 - nu)), 3 * nu
+    for k in range(3):
+        want = _corner_block_by_extrapolation(pts, k, m, params)
+        scale = np.abs(want).max()
+        np.testing.assert_allclose(got[k].sum(axis=1), want.sum(axis=1), rtol=0,
+                                   atol=1e-6 * scale)
+        # the I4 part the reference drops: jac sum_t w CsH0 CsH3 mn phi-weighted I4
+        V = pts
+        nvec = np.cross(V[1] - V[0], V[2] - V[0])
+        jac = np.linalg.norm(nvec)
+        nvec /= jac
+        v0, e1, e2 = V[k], V[(k + 1) % 3] - V[k], V[(k + 2) % 3] - V[k]
+        i4 = np.zeros((3, 3, 3))
+        for t, w in zip(qt, qw):
+            ct, st = np.cos(t), np.sin(t)
+            R = 1.0 / (ct + st)
+            a = e1 * ct + e2 * st
+            A = np.linalg.norm(a)
+            I4 = np.outer(a, a) * fJ45(A, R) - (np.outer(a, nvec) + np.outer(nvec, a)) * \
+                fe35(A, R) + np.outer(nvec, nvec) * fe25(A, R)
+            # local basis (vertex k, k+1, k+2) weights of the rho^2 moment
+            for lb, c in ((0, -(ct + st)), (1, ct), (2, st)):
+                i4[:, (k + lb) % 3, :] += w * CsH0 * CsH3 * (m @ nvec) * c * I4
+        np.testing.assert_allclose(got[k] + jac * i4, want, rtol=0, atol=1e-6 * scale)
+        # the dropped I4 part is no rounding-level effect
+        assert np.abs(jac * i4).max() > 1e-2 * scale
+
+
+@pytest.mark.parametrize('K', ['elasticU3', 'elasticT3'])
+def test_edge_adjacent_numbering_invariance(oracle, K):
+    """One edge-adjacent pair with the source triangle numbered all six
+    ways (three rotations x two orientations), clicks and flip from
+    resolve_ea_rotation.  The local triangles, and so the integrals before
+    de-rotation, are the same for every numbering, so the block per source
+    vertex is fixed by the mathematics: equal for every numbering (for T
+    the reversed numbering reverses the source normal, so the block changes
+    sign).  tectosaur/assemble.cl:85-89 de-rotates a flipped source by
+    swapping entries 0 and 1 of the rotation map, out[b] = local[(f(b) - c)
+    mod 3] with f the 0<->1 swap, where the vertex order needs
+    local[f((b - c) mod 3)]; the two agree when c = 0.  The oracle restates
+    the reference, so its output must be the mathematically fixed block
+    permuted by exactly that deviation."""
+    from tectosaur_amd.nearfield_op import resolve_ea_rotation
+    params = [3e10, 0.25]
+    pts = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.3, 0.9, 0.1], [0.6, -0.8, 0.2]])
+    obs = [0, 1, 2]
+    canon = [1, 0, 3]  # the source traverses the shared edge opposite to the observer
+    qx, qw = edge_adj_quad(4)
+
+    def run(src):
+        tris = np.array([obs, src], dtype=np.int32)
+        sv = [src.index(0), src.index(1)]
+        ea = np.array([[0, 1, 0, sv[0], 1, sv[1]]])
+        pr = resolve_ea_rotation(tris, ea)
+        blk = oracle.pairs_integrate(K, qx, qw, pts, tris, pr, True, params)[0]
+        return blk, int(pr[0, 3]), int(pr[0, 4])
+
+    ref, c0, f0 = run(canon)
+    assert f0 == 0
+    by_vertex = {canon[b]: ref[:, :, b, :] for b in range(3)}
+    swap = [1, 0, 2]
+    deviations = 0
+    for orient in (canon, [canon[1], canon[0], canon[2]]):
+        for rot in range(3):
+            src = [orient[(i + rot) % 3] for i in range(3)]
+            blk, c, flip = run(src)
+            sign = -1.0 if (flip and K == 'elasticT3') else 1.0
+            for b in range(3):
+                if flip:
+                    loc = (swap[b] - c) % 3                  # the reference's map
+                    vert = src[(swap[loc] + c) % 3]          # the vertex local `loc` holds
+                    deviations += vert != src[b]
+                else:
+                    vert = src[b]
+                np.testing.assert_allclose(blk[:, :, b, :], sign * by_vertex[vert], rtol=0,
+                                           atol=1e-13 * np.abs(ref).max())
+    # the deviation is real: some flipped numbering with c != 0 moves a column
+    assert deviations > 0
