@@ -45,9 +45,7 @@ TRAFFIC_SOURCE = 'profiles/far_p2p_H_two_surface_442k_pmc_traffic.md'
 
 def cfg2_mesh():
     from tectosaur_amd import mesh
-    pts, tris, n_surf = mesh.fault_surface_mesh(257, 129, 65, surf_w=20000.0, fault_L=10000.0,
-                                                dip_deg=60.0)
-    return pts, tris
+    return mesh.two_surface_mesh()
 
 
 def time_matvec(op, v_dev, prec, steps, warmup):
@@ -270,19 +268,8 @@ def pairfind_bench(pts, tris, thr=2.5, reps=3):
 
 
 def large_mesh():
-    """The large-solve mesh: a 40 km free surface of 512 x 512 squares with
-    sinusoidal topography (amplitude 500 m, wavelength 10 km; zero along
-    the fault trace) plus a 60-degree dipping fault of 256 x 256 squares
-    whose top edge lies in the surface: 655,360 triangles."""
     from tectosaur_amd import mesh
-    pts, tris, n_surf = mesh.fault_surface_mesh(513, 257, 257, surf_w=20000.0,
-                                                fault_L=10000.0, dip_deg=60.0)
-    surf = np.zeros(pts.shape[0], dtype=bool)
-    surf[np.unique(tris[:n_surf])] = True
-    lam = 10000.0
-    pts[surf, 2] += 500.0 * np.sin(2 * np.pi * pts[surf, 0] / lam) * \
-        np.sin(2 * np.pi * pts[surf, 1] / lam)
-    return pts, tris
+    return mesh.fault_topography_mesh()
 
 
 def large_solve_bench(mode='tree', tol=1e-8):
@@ -343,26 +330,58 @@ def large_solve_bench(mode='tree', tol=1e-8):
     return out
 
 
-def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows):
-    """The CPU oracle (C restatement of farfield_tris_to_tris, FP64 with
-    long-double accumulation, OpenMP) on a bounded sample of observation
-    triangles.  Also returns the GPU FP64/FP32 error on those rows."""
-    import subprocess
+def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows, seconds=8.0):
+    """The reported CPU baseline: the H-kernel point N-body of
+    tectosaur/farfield_direct.cl:15-113 restated for a many-core CPU
+    (oracle/cpu_farfield.c, -O3 -march=native -ffast-math, OpenMP over all
+    host threads), FP32 and FP64, built here for this CPU and timed on a
+    bounded sample of the workload's observation quadrature points against
+    all 442,368 source quadrature points (the strengths the device N-body
+    sums: J w phi-weighted slip).  The FP64 oracle (long double) checks the
+    GPU FP32 and FP64 matvecs on ``rows``."""
+    from oracle import cpu_baseline as CB
     from oracle import oracle as O
     from tectosaur_amd.quadrature import gauss2d_tri
-    if not os.path.exists(os.path.join(ROOT, 'oracle', 'liboracle.so')):
-        subprocess.run(['make', '-C', os.path.join(ROOT, 'oracle')], check=True)
     qx, qw = gauss2d_tri(1)
-    cores = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    # quadrature points, normals and strengths (matrix_free.cl:50-90 order)
+    phi = np.stack([1 - qx[:, 0] - qx[:, 1], qx[:, 0], qx[:, 1]], axis=1)
+    tv = pts[tris]
+    un = np.cross(tv[:, 2] - tv[:, 0], tv[:, 2] - tv[:, 1])
+    jac = np.linalg.norm(un, axis=1)
+    nrm = un / jac[:, None]
+    qp = np.einsum('qb,tbd->tqd', phi, tv).reshape((-1, 3))
+    qn = np.repeat(nrm, qx.shape[0], axis=0)
+    vin = v.reshape((-1, 3, 3))
+    s = np.einsum('t,q,qb,tbd->tqd', jac, qw, phi, vin).reshape((-1, 3))
+    lib = CB.build()
+    eng = CB.HFarfieldCPU(lib, qp, qn, s, PARAMS)
+    rng = np.random.default_rng(5)
+    sample = rng.permutation(qp.shape[0])
+    res = {}
+    for dt in (np.float32, np.float64):
+        rate, n_obs, secs = eng.rate(qp[sample], qn[sample], dt, seconds / 2)
+        res[np.dtype(dt).name] = (rate, n_obs, secs)
+    # the two precisions against each other on the common prefix
+    m = min(res['float32'][1], res['float64'][1], 256)
+    y32 = eng.run(qp[sample[:m]], qn[sample[:m]], np.float32).astype(np.float64)
+    y64 = eng.run(qp[sample[:m]], qn[sample[:m]], np.float64)
+    model, ncpu = CB.cpu_model()
+    cores = int(os.environ.get('OMP_NUM_THREADS', ncpu or 1))
     t0 = time.perf_counter()
     ref = O.farfield_tris('elasticH3', qx, qw, pts, tris, tris, v, PARAMS, rows=rows)
-    dt = time.perf_counter() - t0
-    pairs = rows.shape[0] * 3 * tris.shape[0] * 3
+    t_or = time.perf_counter() - t0
     rl2 = lambda a: float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
-    return {'value': pairs / dt, 'unit': 'pair-evals/s', 'cores': cores, 'kind': 'port',
-            'sample': '%d observation triangles x all %d source triangles (H, FP64 oracle,'
-                      ' %.1f s)' % (rows.shape[0], tris.shape[0], dt)}, \
-        {'f64_vs_oracle_rel_l2': rl2(out64_rows), 'f32_vs_oracle_rel_l2': rl2(out32_rows)}
+    base = {'value': res['float32'][0], 'unit': 'pair-evals/s', 'cores': cores, 'kind': 'port',
+            'sample': '%d observation quadrature points x all %d source points (H, FP32, '
+                      '%.1f s); FP64: %d points, %.1f s' % (
+                          res['float32'][1], qp.shape[0], res['float32'][2],
+                          res['float64'][1], res['float64'][2]),
+            'f64_pair_evals_per_s': res['float64'][0],
+            'cpu_model': model, 'logical_cpus': ncpu,
+            'build': '-O3 -march=native -ffast-math -fopenmp (oracle/cpu_farfield.c)',
+            'f32_vs_f64_rel_l2': float(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))}
+    return base, {'f64_vs_oracle_rel_l2': rl2(out64_rows), 'f32_vs_oracle_rel_l2': rl2(out32_rows),
+                  'oracle_rows': int(rows.shape[0]), 'oracle_s': t_or}
 
 
 def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
@@ -413,7 +432,7 @@ def main():
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-rows', type=int, default=4096)
+    ap.add_argument('--cpu-rows', type=int, default=512)
     ap.add_argument('--kernels', default='elasticU3,elasticT3,elasticA3,elasticH3')
     ap.add_argument('--serve-requests', type=int, default=256)
     ap.add_argument('--serve-modes', default='tree',

@@ -161,3 +161,28 @@ def fault_surface_mesh(n_surf, n_fault_x, n_fault_z, surf_w=20000.0, fault_L=100
                                              [fault_L, dy, dz], [-fault_L, dy, dz]])
     pts, tris = concat(surf, fault)
     return pts, tris, surf[1].shape[0]
+
+
+def two_surface_mesh():
+    """The two-surface farfield workload: a 40 km x 40 km free surface of
+    256 x 256 squares (131,072 triangles) plus a 60-degree dipping planar
+    fault of 128 x 64 squares (16,384 triangles) whose top edge lies in the
+    surface, concatenated with duplicate points merged: 147,456 triangles."""
+    pts, tris, n_surf = fault_surface_mesh(257, 129, 65, surf_w=20000.0, fault_L=10000.0,
+                                           dip_deg=60.0)
+    return pts, tris
+
+
+def fault_topography_mesh():
+    """The large-solve mesh: a 40 km free surface of 512 x 512 squares with
+    sinusoidal topography (amplitude 500 m, wavelength 10 km; zero along
+    the fault trace) plus a 60-degree dipping fault of 256 x 256 squares
+    whose top edge lies in the surface: 655,360 triangles."""
+    pts, tris, n_surf = fault_surface_mesh(513, 257, 257, surf_w=20000.0, fault_L=10000.0,
+                                           dip_deg=60.0)
+    surf = np.zeros(pts.shape[0], dtype=bool)
+    surf[np.unique(tris[:n_surf])] = True
+    lam = 10000.0
+    pts[surf, 2] += 500.0 * np.sin(2 * np.pi * pts[surf, 0] / lam) * \
+        np.sin(2 * np.pi * pts[surf, 1] / lam)
+    return pts, tris

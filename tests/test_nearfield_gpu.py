@@ -125,7 +125,7 @@ def test_nearfield_op_vs_oracle(oracle, Kn, Kf):
     rows, cols, data = [], [], []
     for name in ('co', 'ea', 'va', 'near'):
         prs, dofs, acc, corr = op.classes[name]
-        rprs, rcorr, racc = ref[name]
+        rprs, rdofs, rcorr, racc = ref[name]
         assert np.array_equal(prs, rprs), name
         ok, err = blockwise_close(acc.cpu().numpy(), racc)
         assert ok, (name, 'uncorrected', err)
@@ -182,7 +182,10 @@ def test_bsr_empty_rows_and_matrix():
 def test_cfg3_icosphere_l5_subset(oracle, Kn, Kf):
     # icosphere level 5 (20,480 triangles), all rules n = 8, near threshold 3:
     # every class assembled on the device; a seeded sample of pairs of each
-    # class is integrated by the oracle and compared elementwise.
+    # class is integrated by the oracle and compared elementwise, both the
+    # accurate blocks and the corrected ones (minus the K_far farfield rule),
+    # and the BSR product of the corrected blocks is compared on sampled
+    # block rows with the oracle's blocks of those rows.
     from tectosaur_amd.nearfield_op import RegularizedNearfieldIntegralOp
     pts, tris = mesh.make_icosphere([0, 0, 0], 1.0, 5)
     n = tris.shape[0]
@@ -199,7 +202,71 @@ def test_cfg3_icosphere_l5_subset(oracle, Kn, Kf):
         check0 = name != 'near'
         want = oracle.pairs_integrate(Kn, rules[name][0], rules[name][1], pts, tris, prs[sel],
                                       check0, PARAMS)
-        got = acc.cpu().numpy()[sel]
+        a = acc.cpu().numpy()
+        got = a[sel]
         ok, err = blockwise_close(got, want)
         assert ok, (name, err)
-        assert np.isfinite(acc.cpu().numpy()).all()
+        assert np.isfinite(a).all()
+        # corrected blocks: the error is measured against the accurate block's
+        # size (the difference of the two rules cancels for near pairs)
+        wcorr = oracle.pairs_integrate(Kf, rules['far'][0], rules['far'][1], pts, tris,
+                                       prs[sel, :2], name == 'co', PARAMS)
+        gc = (acc - corr).cpu().numpy()[sel].reshape((-1, 81))
+        wc = (want - wcorr).reshape((-1, 81))
+        scale = np.abs(want.reshape((-1, 81))).max(axis=1)
+        assert (np.abs(gc - wc).max(axis=1) <= 1e-10 * scale).all(), name
+    # BSR product on sampled block rows
+    v = rng.normal(size=9 * n)
+    y = op.dot(v)
+    rows = np.sort(rng.choice(n, size=8, replace=False))
+    for r in rows:
+        yr = np.zeros(9)
+        for name in ('co', 'ea', 'va', 'near'):
+            prs, dofs, _, _ = op.classes[name]
+            mine = np.nonzero(dofs[:, 0] == r)[0]
+            if mine.size == 0:
+                continue
+            check0 = name != 'near'
+            B = oracle.pairs_integrate(Kn, rules[name][0], rules[name][1], pts, tris,
+                                       prs[mine], check0, PARAMS)
+            B = B - oracle.pairs_integrate(Kf, rules['far'][0], rules['far'][1], pts, tris,
+                                           prs[mine, :2], name == 'co', PARAMS)
+            B = B.reshape((-1, 9, 9))
+            for k, j in enumerate(dofs[mine, 1]):
+                yr += B[k] @ v[9 * j:9 * j + 9]
+        assert np.linalg.norm(y[9 * r:9 * r + 9] - yr) <= 1e-10 * np.linalg.norm(yr), r
+
+
+@pytest.mark.parametrize('accumulate', [False, True])
+def test_bsr_long_rows_vs_dense(accumulate):
+    """Block rows of 0, 1, 62, 63, 64, 126, 127 and 200 blocks: the kernel
+    stages the column indices of 63 blocks at a time, so rows longer than
+    63 run several batches.  FP64 blocks and the FP32 copy, with and
+    without accumulation, against the dense product."""
+    import torch
+    from tectosaur_amd.sparse import BCOOMatrix
+    rng = np.random.default_rng(17)
+    lens = [0, 1, 62, 63, 64, 126, 127, 200, 5]
+    nbc = 220
+    rows, cols = [], []
+    for r, L in enumerate(lens):
+        c = np.sort(rng.choice(nbc, size=L, replace=False))
+        rows.append(np.full(L, r))
+        cols.append(c)
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    data = rng.normal(size=(rows.shape[0], 9, 9))
+    shape = (9 * len(lens), 9 * nbc)
+    A = np.zeros(shape)
+    for r, c, d in zip(rows, cols, data):
+        A[9 * r:9 * r + 9, 9 * c:9 * c + 9] = d
+    x = rng.normal(size=shape[1])
+    y0 = rng.normal(size=shape[0])
+    B = BCOOMatrix(rows, cols, data, shape).to_bsr().make_f32()
+    xd = torch.as_tensor(x, device='cuda')
+    A32 = A.astype(np.float32).astype(np.float64)
+    for prec, M in (('f64', A), ('f32', A32)):
+        out = torch.as_tensor(y0.copy(), device='cuda')
+        B.dot_dev(xd, out, accumulate=accumulate, prec=prec)
+        want = (y0 if accumulate else 0) + M @ x
+        np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-12,
+                                   atol=1e-12 * np.abs(want).max())

@@ -27,7 +27,7 @@ def oracle_apply(oracle, pts, tris, u, nu=0.25, K_far='elasticRH3'):
                                      o['near_threshold'], 'elasticRH3', K_far, [1.0, nu],
                                      rules)
     y = np.zeros(tris.shape[0] * 9)
-    for prs, corr, _ in blocks.values():
+    for prs, _, corr, _ in blocks.values():
         y += oracle.bcoo_matvec(prs[:, 0], prs[:, 1], corr, u, (y.shape[0], y.shape[0]))
     qx, qw = gauss2d_tri(o['nq_far'])
     y += oracle.farfield_tris(K_far, qx, qw, pts, tris, tris, u, [1.0, nu])
