@@ -74,6 +74,16 @@ class BSRMatrix:
     def to_dense(self):
         return self.to_scipy().toarray()
 
+    def to_dense_dev(self):
+        """Dense FP64 device tensor of the matrix (blocks summed per slot)."""
+        nbr, nbc = self.shape[0] // 9, self.shape[1] // 9
+        counts = (self.indptr[1:] - self.indptr[:-1]).to(torch.int64)
+        rows = torch.repeat_interleave(torch.arange(nbr, device=self.device), counts)
+        out = torch.zeros((nbr, nbc, 9, 9), dtype=torch.float64, device=self.device)
+        out.index_put_((rows, self.indices.to(torch.int64)), self.data.reshape((-1, 9, 9)),
+                       accumulate=True)
+        return out.permute(0, 2, 1, 3).reshape(self.shape)
+
     @property
     def algorithmic_bytes(self):
         """HBM bytes one matvec must move: block data and column index per
