@@ -134,20 +134,25 @@ int tct_far_gather_curl(int kernel, int prec, int64_t n_pts, int nq, const int32
  * the chunk's source points of K(y_j - x_i) s_j, pairs with r2 == 0
  * skipped.  TCT_F32: FP32 evaluation and FP32 accumulation within each
  * 64-point source tile, FP64 accumulation across tiles; pairs with
- * r2 <= eps2 are skipped, where eps2 bounds the rounding of one point stored
- * relative to two different group origins: (8 * 2^-24 * (E_obs + E_src))^2,
- * E the largest distance of a point from its packed group origin.  Unless
- * cfg->flags bit 0 is set, tiles are freed from the test on the assumption
- * that eps2 is at least that value (a smaller nonzero eps2 needs bit 0;
- * eps2 = 0 tests only r2 == 0, on every tile).  TCT_F64: FP64
- * throughout, absolute coordinates, exact r2 == 0 test (eps2 ignored).
+ * r2 <= eps2 are skipped.  The packing uses eps2 = (8 * 2^-24 *
+ * group_extent)^2, a bound on the rounding of one point stored relative to
+ * two different group origins, so a point paired with itself is skipped.
+ * group_extent = E_obs + E_src, E the largest distance of a point from its
+ * packed group origin: source tiles whose origin is farther than
+ * group_extent + 5 sqrt(eps2) (with a rounding margin) from the wave's
+ * observation group origins hold no pair with r2 <= eps2 and run without
+ * the test (same results).  A negative group_extent, eps2 = 0 or cfg->flags
+ * bit 0 keeps the test on every tile (eps2 = 0 tests r2 == 0 only).
+ * TCT_F64: FP64 throughout, absolute coordinates, exact r2 == 0 test (eps2
+ * and group_extent ignored).
  * Kernels U, T, A, H (3 sums per point) and the surface-curl forms RT (3),
  * RA and RH (9 moments per point, applied to the observation curls by
  * tct_far_scatter_curl); out_partial is [src_chunks][3 or 9][n_obs_pad]. */
 int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float *loc32,
                 const double *obs_org, const double *obs_soa64, int64_t n_src_pad,
                 const void *rec, const double *src_org, const double *params_host,
-                double eps2, const tct_tile_cfg *cfg, double *out_partial, void *stream);
+                double eps2, double group_extent, const tct_tile_cfg *cfg,
+                double *out_partial, void *stream);
 
 /* Sum of the chunk partials: out [n][3] f64 (point order of the packing). */
 int tct_far_reduce_pts(int64_t n, int64_t n_obs_pad, int chunks, const double *partial,

@@ -50,7 +50,8 @@ SIGNATURES = {
                              _vp],
     'tct_far_strength_pts': [_int, _int, _i64, _vp, _vp, _vp, _vp],
     'tct_far_gather_tris': [_int, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
-    'tct_far_p2p': [_int, _int, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _dbl, _vp, _vp, _vp],
+    'tct_far_p2p': [_int, _int, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp,
+                    _vp],
     'tct_far_reduce_pts': [_i64, _i64, _int, _vp, _vp, _vp],
     'tct_tree_p2m': [_int, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     'tct_tree_m2m': [_int, _int, _int, _int, _int, _int, _int, _vp, _vp, _vp],

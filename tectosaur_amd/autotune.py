@@ -111,9 +111,11 @@ def tune(engine, prec, reps=3, save=True, verbose=False):
     """Time every candidate launch shape on ``engine`` (a FarfieldEngine with
     strengths already set), the N-body and the reduction of its chunk
     partials together, and store the fastest.  Returns (cfg, seconds)."""
-    n_cu = torch.cuda.get_device_properties(engine.device).multi_processor_count
+    # candidates come from the fixed CU count of default_tile_cfg (MI355X:
+    # 256), not the device's: the chunk count sets the summation order, so
+    # a table tuned on any box gives the same results on every box
     best = None
-    for block, per_lane, chunks in candidates(engine.n_obs_pad, engine.n_src_pad, n_cu):
+    for block, per_lane, chunks in candidates(engine.n_obs_pad, engine.n_src_pad):
         cfg = N.TileCfg(block, per_lane, chunks, 0)
         _run_and_reduce(engine, prec, cfg)  # warm-up / allocation
         torch.cuda.synchronize()

@@ -650,10 +650,11 @@ extern "C" int tct_far_gather_curl(int kernel, int prec, int64_t n_pts, int nq,
 extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float *loc32,
                            const double *obs_org, const double *obs_soa64, int64_t n_src_pad,
                            const void *rec, const double *src_org, const double *params_host,
-                           double eps2, const tct_tile_cfg *cfg, double *out_partial,
-                           void *stream) {
+                           double eps2, double group_extent, const tct_tile_cfg *cfg,
+                           double *out_partial, void *stream) {
     TCT_CHECK_ARG(kernel >= TCT_U && kernel <= TCT_RH, "far_p2p: kernel id %d", kernel);
-    TCT_CHECK_ARG(eps2 >= 0.0, "far_p2p: eps2 must be >= 0");
+    TCT_CHECK_ARG(eps2 >= 0.0 && std::isfinite(eps2), "far_p2p: eps2 must be finite and >= 0");
+    TCT_CHECK_ARG(!std::isnan(group_extent), "far_p2p: group_extent is NaN");
     TCT_CHECK_ARG(n_obs_pad > 0 && n_obs_pad % G == 0 && n_src_pad > 0 && n_src_pad % G == 0,
                   "far_p2p: sizes must be positive multiples of %d", G);
     TCT_CHECK_ARG(n_obs_pad < (1LL << 31) && n_src_pad < (1LL << 31), "far_p2p: too many points");
@@ -673,16 +674,17 @@ extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float 
               (unsigned)c.src_chunks);
     hipStream_t st = as_stream(stream);
     const int no = (int)n_obs_pad;
-    // eps2 is (8 u (E_obs + E_src))^2 with u = 2^-24 and E the largest
-    // distance of a point from its group origin, so points of groups whose
-    // origins are d apart are at least d - (E_obs + E_src) apart; beyond
-    // sep (with room for the rounding of the local coordinates) no pair can
-    // be skipped by the test.  eps2 = 0, or bit 0 of cfg.flags, keeps
-    // the test everywhere.
+    // group_extent = E_obs + E_src, E the largest distance of a point from
+    // its group origin, so points of groups whose origins are d apart are at
+    // least d - group_extent apart; beyond sep (with room for the rounding
+    // of the local coordinates) no pair can reach r2 <= eps2 and the tile
+    // runs without the test.  The bound uses only the caller's geometry, so
+    // any eps2 is safe.  eps2 = 0, a negative group_extent or bit 0 of
+    // cfg.flags keeps the test everywhere.
     double sep2 = INFINITY;
-    if (eps2 > 0.0 && !(c.flags & 1)) {
+    if (eps2 > 0.0 && group_extent >= 0.0 && std::isfinite(group_extent) && !(c.flags & 1)) {
         const double e = std::sqrt(eps2);
-        const double sep = e / (8.0 * 0x1p-24) * (1.0 + 1e-3) + 4.0 * e;
+        const double sep = group_extent * (1.0 + 1e-3) + 5.0 * e;
         sep2 = sep * sep;
     }
 #define TCT_F32_CASE(KID, P)                                                                 \

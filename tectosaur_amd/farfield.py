@@ -124,7 +124,8 @@ class FarfieldEngine:
         # rounding bound of one point stored against two different origins
         if extents is None:
             extents = (group_extent(obs_pts), group_extent(src_pts))
-        self.eps2 = (8.0 * U32 * (extents[0] + extents[1])) ** 2
+        self.group_extent = float(extents[0] + extents[1])
+        self.eps2 = (8.0 * U32 * self.group_extent) ** 2
         self.tile_cfg = {}
         for prec in ('f32', 'f64'):
             cfg = tile_cfg.get(prec) if isinstance(tile_cfg, dict) else tile_cfg
@@ -187,7 +188,8 @@ class FarfieldEngine:
         N.call('tct_far_p2p', self.kid, self._prec_id(prec), self.n_obs_pad, N.ptr(self.loc32),
                N.ptr(self.obs_org), N.ptr(self.obs_soa64), self.n_src_pad,
                N.ptr(self.rec(prec)), N.ptr(self.src_org), self.params.ctypes.data,
-               float(self.eps2), ctypes_ref(cfg), N.ptr(out), N.stream_ptr())
+               float(self.eps2), self.group_extent, ctypes_ref(cfg), N.ptr(out),
+               N.stream_ptr())
         return out
 
     def reduce_pts(self, prec, out_dev, partial=None, chunks=None):

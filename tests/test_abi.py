@@ -36,16 +36,24 @@ def test_argument_errors_are_reported_without_a_device():
     # argument validation runs on the host before any launch
     from tectosaur_amd import _native as N
     L = N.lib()
-    rc = L.tct_far_p2p(9, 0, 64, None, None, None, 64, None, None, None, 0.0, None, None, None)
+    rc = L.tct_far_p2p(9, 0, 64, None, None, None, 64, None, None, None, 0.0, 0.0, None, None,
+                       None)
     assert rc < 0
     assert b'kernel' in L.tct_last_error()
     # unknown launch-shape flag bits are refused before anything runs
     bad = N.TileCfg(256, 1, 1, 2)
     buf = ctypes.c_double(0.0)
     rc = L.tct_far_p2p(3, 0, 64, ctypes.byref(buf), ctypes.byref(buf), None, 64,
-                       ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf), 0.0,
+                       ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf), 0.0, 0.0,
                        ctypes.byref(bad), ctypes.byref(buf), None)
     assert rc < 0 and b'flags' in L.tct_last_error()
+    # a NaN extent or a non-finite eps2 is refused
+    ok = N.TileCfg(256, 1, 1, 0)
+    for eps2, ext in ((float('inf'), 0.0), (1e-12, float('nan'))):
+        rc = L.tct_far_p2p(3, 0, 64, ctypes.byref(buf), ctypes.byref(buf), None, 64,
+                           ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf), eps2, ext,
+                           ctypes.byref(ok), ctypes.byref(buf), None)
+        assert rc < 0 and (b'eps2' in L.tct_last_error() or b'extent' in L.tct_last_error())
     n = ctypes.c_int64(-1)
     rc = L.tct_find_near_pairs(0, None, None, 0, None, None, 1.0, None, 0, ctypes.byref(n))
     assert rc == 0 and n.value == 0

@@ -237,3 +237,25 @@ def test_tiles_without_the_r2_test_change_nothing(K):
             outs.append(op.dot(v))
         assert np.all(np.isfinite(outs[0]))
         assert np.array_equal(outs[0], outs[1]), (K, obs.shape[0], src.shape[0])
+
+
+@pytest.mark.parametrize('K', ['elasticH3', 'elasticRH3'])
+def test_tile_skip_is_safe_for_any_eps2(K):
+    # the tiles that skip the r2 test are chosen from the group extents the
+    # caller passes, not from eps2, so an eps2 far below the packing bound
+    # still tests every tile that can hold a pair at r2 <= eps2: the same
+    # bits as the test on every tile, and finite (a point paired with itself
+    # in one packing has r2 == 0 exactly)
+    from tectosaur_amd import _native as N
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = cfg1_mesh()
+    n = tris.shape[0]
+    outs = []
+    for flags in (0, 1):
+        op = TriToTriDirectFarfieldOp(2, K, [3e10, 0.25], pts, tris, np.float32, np.arange(n),
+                                      np.arange(n), tile_cfg=N.TileCfg(128, 1, 4, flags))
+        op.engine.eps2 = 1e-30
+        v = np.random.default_rng(4).normal(size=op.shape[1])
+        outs.append(op.dot(v))
+    assert np.all(np.isfinite(outs[0]))
+    assert np.array_equal(outs[0], outs[1])
