@@ -53,7 +53,11 @@ enum tct_prec { TCT_F32 = 0, TCT_F64 = 1 };
  *   flags:        bit 0 keeps the FP32 kernel's r2 <= eps2 test on every
  *                 source tile (by default tiles too far from a wave's
  *                 observation groups to hold such a pair skip it; the
- *                 results are the same); other bits must be 0. */
+ *                 results are the same); bit 1 orders the workgroups by
+ *                 XCD (a 1-D grid; src_chunks a multiple of 8; XCD x works
+ *                 through chunks x, x + 8, ... for every observation block,
+ *                 so one chunk of source records at a time stays in each
+ *                 XCD's L2; the same results); other bits must be 0. */
 typedef struct {
     int32_t block;
     int32_t obs_per_lane;
@@ -189,7 +193,7 @@ int tct_far_scatter_curl(int kernel, int64_t n_tris, int nq, const int32_t *pt_o
  *   moments [n_nodes][C][n_terms(order)] in the precision of the records,
  *     C = 4 (U, RA), 13 (RT), 12 (RH), n_terms(p) = (p+1)(p+2)(p+3)/6.
  * Kernels: TCT_U, TCT_RT, TCT_RA, TCT_RH; orders 0..TCT_TREE_MAX_ORDER for
- * p2m/m2m, and 2-6 for eval (others return TCT_EUNSUPPORTED). */
+ * p2m/m2m, and 2-8 for eval (others return TCT_EUNSUPPORTED). */
 #define TCT_TREE_MAX_ORDER 8
 
 /* Tile moments (nodes 0..n_tiles-1) from the packed source records (the

@@ -1,6 +1,7 @@
 """Launch-shape autotuner for the farfield N-body kernel.
 
-The tunable shape is ``TileCfg(block, obs_per_lane, src_chunks)``.  A tuning
+The tunable shape is ``TileCfg(block, obs_per_lane, src_chunks, flags)``
+with flags 0 or 2 (the XCD chunk order, src_chunks a multiple of 8).  A tuning
 run times every candidate on the device for one (kernel, precision, size
 class) and records the fastest in a JSON table (``autotune_table.json``
 next to this file, or the path in TCT_AUTOTUNE_TABLE).  ``lookup`` reads the
@@ -78,7 +79,8 @@ def lookup(K_name, prec, n_obs_pad, n_src_pad, device=None):
     if e is not None:
         n_tiles = n_src_pad // GROUP
         chunks = max(1, min(int(e['src_chunks']), n_tiles))
-        return N.TileCfg(int(e['block']), int(e['obs_per_lane']), chunks, 0)
+        flags = 2 if (e.get('xcd') and chunks % 8 == 0) else 0
+        return N.TileCfg(int(e['block']), int(e['obs_per_lane']), chunks, flags)
     return default_tile_cfg(n_obs_pad, n_src_pad, prec, device)
 
 
@@ -91,9 +93,13 @@ def candidates(n_obs_pad, n_src_pad, n_cu=256):
             obs_blocks = -(-n_obs_pad // (block * per_lane))
             for waves_target in (4, 8, 16, 32):
                 chunks = max(1, min(n_tiles, -(-waves_target * n_cu // obs_blocks)))
-                cfg = (block, per_lane, chunks)
+                cfg = (block, per_lane, chunks, 0)
                 if cfg not in out:
                     out.append(cfg)
+            # the XCD chunk order: 8, 16, 24 or 32 chunks
+            for chunks in (8, 16, 24, 32):
+                if chunks <= n_tiles:
+                    out.append((block, per_lane, chunks, 2))
     return out
 
 
@@ -115,8 +121,8 @@ def tune(engine, prec, reps=3, save=True, verbose=False):
     # 256), not the device's: the chunk count sets the summation order, so
     # a table tuned on any box gives the same results on every box
     best = None
-    for block, per_lane, chunks in candidates(engine.n_obs_pad, engine.n_src_pad):
-        cfg = N.TileCfg(block, per_lane, chunks, 0)
+    for block, per_lane, chunks, flags in candidates(engine.n_obs_pad, engine.n_src_pad):
+        cfg = N.TileCfg(block, per_lane, chunks, flags)
         _run_and_reduce(engine, prec, cfg)  # warm-up / allocation
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -125,8 +131,8 @@ def tune(engine, prec, reps=3, save=True, verbose=False):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         if verbose:
-            print('tune %s %s block=%d P=%d chunks=%d: %.3f ms' % (
-                engine.K.name, prec, block, per_lane, chunks, dt * 1e3), flush=True)
+            print('tune %s %s block=%d P=%d chunks=%d flags=%d: %.3f ms' % (
+                engine.K.name, prec, block, per_lane, chunks, flags, dt * 1e3), flush=True)
         if best is None or dt < best[1]:
             best = (cfg, dt)
     engine.tile_cfg[prec] = best[0]
@@ -136,7 +142,8 @@ def tune(engine, prec, reps=3, save=True, verbose=False):
         with _LOCK:
             t[key] = {
                 'block': best[0].block, 'obs_per_lane': best[0].obs_per_lane,
-                'src_chunks': best[0].src_chunks, 'seconds': best[1]}
+                'src_chunks': best[0].src_chunks, 'xcd': bool(best[0].flags & 2),
+                'seconds': best[1]}
             try:
                 with open(table_path(), 'w') as f:
                     json.dump(t, f, indent=1, sort_keys=True)

@@ -26,7 +26,19 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=False, jobs=None):
+def _stale(obj, src):
+    """True when obj is missing or older than its source or any header."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src, os.path.join(HERE, '..', 'include', 'tct_hip.h')]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.h')]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose=False, jobs=None, incremental=False):
+    """Compile every source (``incremental``: only objects older than their
+    source or a header) and link libtct_hip.so."""
     hipcc = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
     bdir = os.path.join(HERE, '_build')
     os.makedirs(bdir, exist_ok=True)
@@ -37,17 +49,21 @@ def build(verbose=False, jobs=None):
                  '-Wno-unused-command-line-argument']
     for src in HIP_SOURCES:
         obj = os.path.join(bdir, src.replace('.hip', '.o'))
+        objs.append(obj)
+        if incremental and not _stale(obj, os.path.join(CSRC, src)):
+            continue
         cmd = [hipcc] + hip_flags + ['-c', os.path.join(CSRC, src), '-o', obj]
         print('+', ' '.join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
-        objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(bdir, src.replace('.cpp', '.o'))
+        objs.append(obj)
+        if incremental and not _stale(obj, os.path.join(CSRC, src)):
+            continue
         cmd = ['g++', '-O3', '-std=c++17', '-fPIC', '-fopenmp', '-ffp-contract=off', '-Wall',
                '-c', os.path.join(CSRC, src), '-o', obj]
         print('+', ' '.join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
-        objs.append(obj)
     bad = [p.args for p in procs if p.wait() != 0]
     if bad:
         raise RuntimeError('compile failed: %s' % bad)

@@ -50,8 +50,28 @@ __device__ __forceinline__ void fence_recs(float (&xs)[P][3], float last) {
         asm volatile("" : "+v"(xs[p][0]), "+v"(xs[p][1]), "+v"(xs[p][2]) : "s"(last) : "memory");
 }
 
+// The (observation block, source chunk) of this workgroup.  Plain order:
+// grid = (observation blocks, chunks).  XCD order (xcd_chunks > 0): a 1-D
+// grid of observation blocks x chunks, chunks a multiple of 8; workgroups
+// are dispatched round-robin over the 8 XCDs by linear id, so workgroup L
+// runs on XCD L mod 8, and XCD x is given chunks x, x + 8, ..., each for
+// every observation block before the next: the workgroups resident on one
+// XCD at a time read one chunk of source records, which stays in that
+// XCD's L2.  Any dispatch order gives the same results (the map is a
+// bijection); only cache reuse depends on it.
+struct BlockChunk {
+    int ob, c, chunks;
+};
+__device__ __forceinline__ BlockChunk block_chunk(int xcd_chunks, int obs_blocks) {
+    if (xcd_chunks == 0) return {(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.y};
+    const int L = (int)blockIdx.x;
+    const int k = L >> 3;
+    return {k % obs_blocks, (k / obs_blocks) * 8 + (L & 7), xcd_chunks};
+}
+
 // ---------------------------------------------------------------------------
-// FP32 N-body.  grid = (ceil(n_obs_pad / (blockDim*P)), chunks).
+// FP32 N-body.  grid = (ceil(n_obs_pad / (blockDim*P)), chunks), or the XCD
+// order of block_chunk.
 // loc32 [6][n_obs_pad]; obs_org [n_obs_pad/G][3]; rec [n_src_pad][RW];
 // src_org [n_src_pad/G][3]; partial [chunks][NO][n_obs_pad].
 // sep2: a source tile whose origin is farther than sqrt(sep2) from the
@@ -62,13 +82,15 @@ template <int KID, int P>
 __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
     int n_obs_pad, const float *__restrict__ loc32, const double *__restrict__ obs_org,
     int n_tiles, const float *__restrict__ rec, const double *__restrict__ src_org,
-    KConst<float> k, double sep2, double *__restrict__ partial) {
+    KConst<float> k, double sep2, double *__restrict__ partial, int xcd_chunks,
+    int obs_blocks) {
     constexpr int RW = rec_words<KID>();
     constexpr int NO = n_out<KID>();
     const int bd = blockDim.x;
-    const int base = blockIdx.x * bd * P;
-    const int chunks = gridDim.y;
-    const int c = blockIdx.y;
+    const BlockChunk bc = block_chunk(xcd_chunks, obs_blocks);
+    const int base = bc.ob * bd * P;
+    const int chunks = bc.chunks;
+    const int c = bc.c;
     const int t0 = (int)(((long)n_tiles * c) / chunks);
     const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
 
@@ -187,13 +209,15 @@ __global__ __launch_bounds__(256) void far_p2p_f64_kernel(int n_obs_pad,
                                                           int n_tiles,
                                                           const double *__restrict__ rec,
                                                           KConst<double> k,
-                                                          double *__restrict__ partial) {
+                                                          double *__restrict__ partial,
+                                                          int xcd_chunks, int obs_blocks) {
     constexpr int RW = rec_words<KID>();
     constexpr int NO = n_out<KID>();
     const int bd = blockDim.x;
-    const int base = blockIdx.x * bd * P;
-    const int chunks = gridDim.y;
-    const int c = blockIdx.y;
+    const BlockChunk bc = block_chunk(xcd_chunks, obs_blocks);
+    const int base = bc.ob * bd * P;
+    const int chunks = bc.chunks;
+    const int c = bc.c;
     const int t0 = (int)(((long)n_tiles * c) / chunks);
     const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
 
@@ -523,16 +547,18 @@ __global__ void scatter_curl_kernel(int64_t n_tris, int nq, const int32_t *__res
 template <int KID, int P>
 static void launch_f32(dim3 grid, int block, hipStream_t st, int n_obs_pad, const float *loc32,
                        const double *obs_org, int n_tiles, const float *rec,
-                       const double *src_org, KConst<float> k, double sep2, double *partial) {
+                       const double *src_org, KConst<float> k, double sep2, double *partial,
+                       int xcd_chunks, int obs_blocks) {
     hipLaunchKernelGGL((far_p2p_f32_kernel<KID, P>), grid, dim3(block), 0, st, n_obs_pad, loc32,
-                       obs_org, n_tiles, rec, src_org, k, sep2, partial);
+                       obs_org, n_tiles, rec, src_org, k, sep2, partial, xcd_chunks, obs_blocks);
 }
 
 template <int KID, int P>
 static void launch_f64(dim3 grid, int block, hipStream_t st, int n_obs_pad, const double *soa64,
-                       int n_tiles, const double *rec, KConst<double> k, double *partial) {
+                       int n_tiles, const double *rec, KConst<double> k, double *partial,
+                       int xcd_chunks, int obs_blocks) {
     hipLaunchKernelGGL((far_p2p_f64_kernel<KID, P>), grid, dim3(block), 0, st, n_obs_pad, soa64,
-                       n_tiles, rec, k, partial);
+                       n_tiles, rec, k, partial, xcd_chunks, obs_blocks);
 }
 
 
@@ -664,14 +690,22 @@ extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float 
     tct_tile_cfg c{256, 2, 1, 0};
     if (cfg) c = *cfg;
     TCT_CHECK_ARG(c.block == 64 || c.block == 128 || c.block == 256, "far_p2p: block %d", c.block);
-    TCT_CHECK_ARG((c.flags & ~1) == 0, "far_p2p: unknown flags 0x%x", (unsigned)c.flags);
+    TCT_CHECK_ARG((c.flags & ~3) == 0, "far_p2p: unknown flags 0x%x", (unsigned)c.flags);
     TCT_CHECK_ARG(c.obs_per_lane == 1 || c.obs_per_lane == 2 || c.obs_per_lane == 4,
                   "far_p2p: obs_per_lane %d", c.obs_per_lane);
     const int n_tiles = (int)(n_src_pad / G);
     TCT_CHECK_ARG(c.src_chunks >= 1 && c.src_chunks <= n_tiles && c.src_chunks <= 65535,
                   "far_p2p: src_chunks %d not in [1, %d]", c.src_chunks, n_tiles);
-    dim3 grid((unsigned)ceil_div(n_obs_pad, (int64_t)c.block * c.obs_per_lane),
-              (unsigned)c.src_chunks);
+    const int64_t obs_blocks = ceil_div(n_obs_pad, (int64_t)c.block * c.obs_per_lane);
+    const bool xcd = (c.flags & 2) != 0;
+    TCT_CHECK_ARG(!xcd || c.src_chunks % 8 == 0,
+                  "far_p2p: XCD chunk order needs src_chunks a multiple of 8 (got %d)",
+                  c.src_chunks);
+    TCT_CHECK_ARG(obs_blocks * c.src_chunks < (1LL << 31), "far_p2p: grid too large");
+    const int xcd_chunks = xcd ? c.src_chunks : 0;
+    const int ob = (int)obs_blocks;
+    dim3 grid = xcd ? dim3((unsigned)(obs_blocks * c.src_chunks), 1u)
+                    : dim3((unsigned)obs_blocks, (unsigned)c.src_chunks);
     hipStream_t st = as_stream(stream);
     const int no = (int)n_obs_pad;
     // group_extent = E_obs + E_src, E the largest distance of a point from
@@ -689,10 +723,11 @@ extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float 
     }
 #define TCT_F32_CASE(KID, P)                                                                 \
     launch_f32<KID, P>(grid, c.block, st, no, loc32, obs_org, n_tiles, (const float *)rec,    \
-                       src_org, kernel_consts<float>(KID, params_host, eps2), sep2, out_partial)
+                       src_org, kernel_consts<float>(KID, params_host, eps2), sep2, out_partial, \
+                       xcd_chunks, ob)
 #define TCT_F64_CASE(KID, P)                                                              \
     launch_f64<KID, P>(grid, c.block, st, no, obs_soa64, n_tiles, (const double *)rec,     \
-                       kernel_consts<double>(KID, params_host, 0.0), out_partial)
+                       kernel_consts<double>(KID, params_host, 0.0), out_partial, xcd_chunks, ob)
 #define TCT_P_SWITCH(MAC, KID)                          \
     switch (c.obs_per_lane) {                           \
     case 1: MAC(KID, 1); break;                         \

@@ -558,8 +558,10 @@ inline int eval_order(int order, int prec, dim3 grid, hipStream_t st, int n_obs_
     case 4: TCT_EV(4); break;
     case 5: TCT_EV(5); break;
     case 6: TCT_EV(6); break;
+    case 7: TCT_EV(7); break;
+    case 8: TCT_EV(8); break;
     default:
-        return set_error(TCT_EUNSUPPORTED, "tree_eval: order %d not built (2-6)", order);
+        return set_error(TCT_EUNSUPPORTED, "tree_eval: order %d not built (2-8)", order);
     }
 #undef TCT_EV
     return TCT_OK;

@@ -59,29 +59,52 @@ __global__ __launch_bounds__(KB) void combine_kernel(int m, int64_t n, int64_t l
     out[t] = (beta == 0.0 ? 0.0 : beta * out[t]) + alpha * s;
 }
 
+// Column sums of the per-block partials, each in block order b = 0..nb-1:
+// c[i] = sum_b partial[b][i], hout[i] = c[i] (+ add[i]).  One thread per
+// column; used when there are many blocks (below).
+__global__ __launch_bounds__(KB) void reduce_cols_kernel(int m, int nb,
+                                                         const double *__restrict__ partial,
+                                                         const double *__restrict__ add,
+                                                         double *__restrict__ c,
+                                                         double *__restrict__ hout) {
+    const int i = blockIdx.x * KB + threadIdx.x;
+    if (i >= m) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += partial[(size_t)b * m + i];
+    c[i] = s;
+    if (hout) hout[i] = add ? s + add[i] : s;
+}
+
 // The reduce, the update and the next sums in one launch, on the partition
-// of multi_dot_kernel (gridDim.x == nb blocks of contiguous points): every
-// block sums partial_in[b][0..m) in fixed order into LDS c, block 0 writes
-// c (+ add) to hout; w[t] -= sum_i V[i][t] c[i] on the block's points; then
-// partial_out[b][i] = sum over those points of V[i][t] w[t] (norm_only: the
-// one column sum w[t]^2).  Each w[t] is updated by one sum in a fixed order
-// and the sums follow multi_dot_kernel's per-thread order, so the values do
-// not depend on the launch shape and match an update launch followed by
-// multi_dot_kernel bit for bit; each thread reads back only the w[t] it
-// wrote.  partial_in and partial_out must not overlap.
+// of multi_dot_kernel (gridDim.x == nb blocks of contiguous points): the
+// column sums c[i] of partial_in[b][0..m) in block order -- summed by every
+// block itself, block 0 writing c (+ add) to hout, or, with many blocks,
+// read from c_in where reduce_cols_kernel left them (the same sums, so the
+// same bits, without nb re-reads of the nb x m partials); w[t] -= sum_i
+// V[i][t] c[i] on the block's points; then partial_out[b][i] = sum over
+// those points of V[i][t] w[t] (norm_only: the one column sum w[t]^2).
+// Each w[t] is updated by one sum in a fixed order and the sums follow
+// multi_dot_kernel's per-thread order, so the values do not depend on the
+// launch shape and match an update launch followed by multi_dot_kernel bit
+// for bit; each thread reads back only the w[t] it wrote.  partial_in and
+// partial_out must not overlap.
 __global__ __launch_bounds__(KB) void reduce_update_dot_kernel(
     int m, int64_t n, int64_t ld, const double *__restrict__ V,
-    const double *__restrict__ partial_in, const double *__restrict__ add,
-    double *__restrict__ hout, double *__restrict__ w, int norm_only,
-    double *__restrict__ partial_out) {
+    const double *__restrict__ partial_in, const double *__restrict__ c_in,
+    const double *__restrict__ add, double *__restrict__ hout, double *__restrict__ w,
+    int norm_only, double *__restrict__ partial_out) {
     extern __shared__ double c[];
     __shared__ double red[KB / 64];
     const int nb = gridDim.x;
     for (int i = threadIdx.x; i < m; i += KB) {
-        double s = 0.0;
-        for (int b = 0; b < nb; ++b) s += partial_in[(size_t)b * m + i];
-        c[i] = s;
-        if (blockIdx.x == 0 && hout) hout[i] = add ? s + add[i] : s;
+        if (c_in) {
+            c[i] = c_in[i];
+        } else {
+            double s = 0.0;
+            for (int b = 0; b < nb; ++b) s += partial_in[(size_t)b * m + i];
+            c[i] = s;
+            if (blockIdx.x == 0 && hout) hout[i] = add ? s + add[i] : s;
+        }
     }
     __syncthreads();
     const int64_t per = (n + nb - 1) / nb;
@@ -158,6 +181,47 @@ inline int n_blocks(int64_t n) {
     return (int)(nb < 1 ? 1 : (nb > 1024 ? 1024 : nb));
 }
 
+// From this many blocks on, the column sums of the partials are formed once
+// by reduce_cols_kernel (two more launches per step) instead of by every
+// block of the update (nb^2 m re-reads).
+constexpr int NB_SEPARATE_REDUCE = 128;
+
+// The four (or six) launches of one CGS2 step; scratch = p1 [nb][m],
+// p2 [nb][m], h1 [m], c [m].
+inline void cgs2_launch(int m, int64_t n, int64_t ld, const double *V, double *w, double *h,
+                        double *scratch, double *vnext, double *vnext2, hipStream_t st) {
+    const int nb = n_blocks(n);
+    double *p1 = scratch;                     // [nb][m], later the norm partials
+    double *p2 = scratch + (size_t)nb * m;    // [nb][m]
+    double *h1 = scratch + 2 * (size_t)nb * m;
+    double *cs = h1 + m;                      // column sums (separate reduce)
+    const bool sep = nb >= NB_SEPARATE_REDUCE;
+    const unsigned gr = (unsigned)ceil_div((int64_t)m, (int64_t)KB);
+    // normalisation launch: enough blocks to cover the points, at most 1024
+    const int64_t gpl = ceil_div(n, (int64_t)KB);
+    const unsigned gp = (unsigned)(gpl < 1024 ? gpl : 1024);
+    const size_t lds = sizeof(double) * (size_t)m;
+    // p1 = partial sums of V w
+    hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, m, n, ld, V, w, 0, p1);
+    // h1 = sum p1; w -= V^T h1; p2 = partial sums of V w
+    if (sep)
+        hipLaunchKernelGGL(reduce_cols_kernel, dim3(gr), dim3(KB), 0, st, m, nb,
+                           (const double *)p1, (const double *)nullptr, cs, h1);
+    hipLaunchKernelGGL(reduce_update_dot_kernel, dim3(nb), dim3(KB), lds, st, m, n, ld, V,
+                       (const double *)p1, (const double *)(sep ? cs : nullptr),
+                       (const double *)nullptr, h1, w, 0, p2);
+    // h2 = sum p2, h = h1 + h2; w -= V^T h2; p1 = partial sums of w^2
+    if (sep)
+        hipLaunchKernelGGL(reduce_cols_kernel, dim3(gr), dim3(KB), 0, st, m, nb,
+                           (const double *)p2, (const double *)h1, cs, h);
+    hipLaunchKernelGGL(reduce_update_dot_kernel, dim3(nb), dim3(KB), lds, st, m, n, ld, V,
+                       (const double *)p2, (const double *)(sep ? cs : nullptr),
+                       (const double *)h1, h, w, 1, p1);
+    // h[m] = |w|, and the next basis vector
+    hipLaunchKernelGGL(norm_normalize_kernel, dim3(vnext ? gp : 1), dim3(KB), 0, st, n, w, p1, nb,
+                       h + m, vnext, vnext2);
+}
+
 }  // namespace tct
 
 using namespace tct;
@@ -165,8 +229,8 @@ using namespace tct;
 extern "C" int tct_krylov_scratch(int m, int64_t n, int64_t *n_doubles) {
     TCT_CHECK_ARG(m >= 1 && n >= 1 && n_doubles, "krylov_scratch: bad arguments");
     // two partial arrays of m columns per block (one read while the other
-    // is written), plus h1 [m]
-    *n_doubles = 2 * (int64_t)n_blocks(n) * m + m;
+    // is written), plus h1 [m] and the column sums [m]
+    *n_doubles = 2 * (int64_t)n_blocks(n) * m + 2 * m;
     return TCT_OK;
 }
 
@@ -177,27 +241,10 @@ extern "C" int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w
                   (long long)n, (long long)ld);
     TCT_CHECK_ARG(V && w && h && scratch, "cgs2: null pointer");
     const int nb = n_blocks(n);
-    TCT_CHECK_ARG(scratch_doubles >= 2 * (int64_t)nb * m + m, "cgs2: scratch too small");
-    hipStream_t st = as_stream(stream);
-    double *p1 = scratch;                     // [nb][m], later the norm partials
-    double *p2 = scratch + (size_t)nb * m;    // [nb][m]
-    double *h1 = scratch + 2 * (size_t)nb * m;
-    // normalisation launch: enough blocks to cover the points, at most 1024
-    const int64_t gpl = ceil_div(n, KB);
-    const unsigned gp = (unsigned)(gpl < 1024 ? gpl : 1024);
-    const size_t lds = sizeof(double) * (size_t)m;
-    TCT_CHECK_ARG(lds <= 60 * 1024, "cgs2: basis size %d too large for LDS", m);
-    // p1 = partial sums of V w
-    hipLaunchKernelGGL(multi_dot_kernel, dim3(nb), dim3(KB), 0, st, m, n, ld, V, w, 0, p1);
-    // h1 = sum p1; w -= V^T h1; p2 = partial sums of V w
-    hipLaunchKernelGGL(reduce_update_dot_kernel, dim3(nb), dim3(KB), lds, st, m, n, ld, V,
-                       (const double *)p1, (const double *)nullptr, h1, w, 0, p2);
-    // h2 = sum p2, h = h1 + h2; w -= V^T h2; p1 = partial sums of w^2
-    hipLaunchKernelGGL(reduce_update_dot_kernel, dim3(nb), dim3(KB), lds, st, m, n, ld, V,
-                       (const double *)p2, (const double *)h1, h, w, 1, p1);
-    // h[m] = |w|, and the next basis vector
-    hipLaunchKernelGGL(norm_normalize_kernel, dim3(vnext ? gp : 1), dim3(KB), 0, st, n, w,
-                       p1, nb, h + m, vnext, vnext2);
+    TCT_CHECK_ARG(scratch_doubles >= 2 * (int64_t)nb * m + 2 * m, "cgs2: scratch too small");
+    TCT_CHECK_ARG(sizeof(double) * (size_t)m <= 60 * 1024, "cgs2: basis size %d too large for LDS",
+                  m);
+    cgs2_launch(m, n, ld, V, w, h, scratch, vnext, vnext2, as_stream(stream));
     TCT_CHECK_LAUNCH("cgs2");
     return TCT_OK;
 }

@@ -32,7 +32,7 @@ from .kernels import kernels as KERNELS
 from .sparse_farfield_op import TriToTriDirectFarfieldOp, farfield_plan
 
 TREE_KERNELS = ('elasticU3', 'elasticRT3', 'elasticRA3', 'elasticRH3')
-EVAL_ORDERS = (2, 3, 4, 5, 6)
+EVAL_ORDERS = (2, 3, 4, 5, 6, 7, 8)
 N_CHANNELS = {'elasticU3': 4, 'elasticRA3': 4, 'elasticRT3': 13, 'elasticRH3': 12}
 
 

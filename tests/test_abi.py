@@ -41,7 +41,7 @@ def test_argument_errors_are_reported_without_a_device():
     assert rc < 0
     assert b'kernel' in L.tct_last_error()
     # unknown launch-shape flag bits are refused before anything runs
-    bad = N.TileCfg(256, 1, 1, 2)
+    bad = N.TileCfg(256, 1, 1, 4)
     buf = ctypes.c_double(0.0)
     rc = L.tct_far_p2p(3, 0, 64, ctypes.byref(buf), ctypes.byref(buf), None, 64,
                        ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf), 0.0, 0.0,

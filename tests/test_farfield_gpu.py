@@ -145,9 +145,12 @@ def test_cfg1_u_farfield(oracle, prec):
     assert rel_l2(out, ref) <= TOL[prec]
 
 
-@pytest.mark.parametrize('cfg', [(64, 1, 1), (128, 2, 3), (256, 4, 7), (256, 1, 32)])
+@pytest.mark.parametrize('cfg', [(64, 1, 1, 0), (128, 2, 3, 0), (256, 4, 7, 0), (256, 1, 32, 0),
+                                 (256, 1, 8, 2), (128, 2, 16, 2), (64, 4, 24, 2)])
 def test_tile_shapes_agree(cfg):
-    # every launch shape computes the same sum; FP64 results agree to roundoff
+    # every launch shape computes the same sum; FP64 results agree to
+    # roundoff; the XCD chunk order (flags 2) gives the bits of the plain
+    # order with the same chunks
     from tectosaur_amd import _native as N
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
     pts, tris = cfg1_mesh()
@@ -157,8 +160,25 @@ def test_tile_shapes_agree(cfg):
                                     np.arange(n), np.arange(n)).dot(v)
     for ft, tol in ((np.float64, 1e-13), (np.float32, 1e-5)):
         op = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, ft, np.arange(n),
-                                      np.arange(n), tile_cfg=N.TileCfg(*cfg, 0))
-        assert rel_l2(op.dot(v), base) <= tol
+                                      np.arange(n), tile_cfg=N.TileCfg(*cfg))
+        out = op.dot(v)
+        assert rel_l2(out, base) <= tol
+        if cfg[3] & 2:
+            plain = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, ft,
+                                             np.arange(n), np.arange(n),
+                                             tile_cfg=N.TileCfg(cfg[0], cfg[1], cfg[2], 0))
+            assert np.array_equal(plain.dot(v), out)
+
+
+def test_xcd_order_needs_chunks_multiple_of_8():
+    from tectosaur_amd import _native as N
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    pts, tris = cfg1_mesh()
+    n = tris.shape[0]
+    op = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, np.float32,
+                                  np.arange(n), np.arange(n), tile_cfg=N.TileCfg(256, 1, 12, 2))
+    with pytest.raises(N.TctError, match='multiple of 8'):
+        op.dot(np.ones(n * 9))
 
 
 def test_deterministic():

@@ -61,3 +61,29 @@ def test_graphs_captured_in_separate_calls_stay_valid():
         op.replay(p)
         torch.cuda.synchronize()
         assert torch.equal(op.gy, eager[p]), p
+
+
+def test_upload_from_builder_threads_keeps_every_copy():
+    # builder threads upload host arrays through pinned memory on streams of
+    # their own without waiting for the device, and drop each host array at
+    # once; torch's pinned blocks are reused only after their copies finish,
+    # so every device copy must equal its source
+    import concurrent.futures
+    from tectosaur_amd import _native as N
+
+    def work(seed):
+        st = torch.cuda.Stream()
+        rng = np.random.default_rng(seed)
+        outs, refs = [], []
+        with torch.cuda.stream(st):
+            for _ in range(40):
+                a = rng.standard_normal(int(rng.integers(1, 150000)))
+                refs.append(a.copy())
+                outs.append(N.upload(a, torch.float64))
+                del a
+                outs[-1].mul_(1.0)   # queued device work behind the copy
+        st.synchronize()
+        return all(np.array_equal(o.cpu().numpy(), r) for o, r in zip(outs, refs))
+
+    with concurrent.futures.ThreadPoolExecutor(4) as ex:
+        assert all(ex.map(work, range(4)))

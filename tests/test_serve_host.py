@@ -1,0 +1,101 @@
+"""Host-side invariants of the server's thread plumbing (CPU, with stand-in
+device objects): graphs are created and destroyed only under the server's
+capture lock, so no graph is destroyed while another thread captures (the
+abort 'operation not permitted when stream is capturing' from a graph
+destructor), and a finished solve's buffers are returned only after its
+build stream has waited for its solve stream."""
+import threading
+
+import numpy as np
+import pytest
+
+from tectosaur_amd import serve
+
+
+class _LockProbe:
+    """A lock that records whether it was held when asked."""
+
+    def __init__(self):
+        self._lk = threading.Lock()
+
+    def __enter__(self):
+        self._lk.acquire()
+        return self
+
+    def __exit__(self, *a):
+        self._lk.release()
+
+    def held(self):
+        return self._lk.locked()
+
+
+class _DelProbe:
+    def __init__(self, lock, log):
+        self.lock, self.log = lock, log
+
+    def __del__(self):
+        self.log.append(self.lock.held())
+
+
+class _Stream:
+    def __init__(self, log, name):
+        self.log, self.name = log, name
+
+    def wait_stream(self, other):
+        self.log.append(('wait', self.name, other.name))
+
+
+def test_release_destroys_graphs_under_the_capture_lock_after_stream_join():
+    lock = _LockProbe()
+    held = []
+    order = []
+    a = serve._Active(None, _DelProbe(lock, held), _DelProbe(lock, held),
+                      _DelProbe(lock, held), 0, stream=_Stream(order, 'solve'),
+                      build_stream=_Stream(order, 'build'))
+    serve._release(a, lock)
+    assert held == [True, True, True]       # every object dropped with the lock held
+    assert order == [('wait', 'build', 'solve')]
+    assert a.op is None and a.state is None and a.prob is None
+    assert not lock.held()
+
+
+def test_build_captures_under_the_capture_lock(monkeypatch):
+    lock = _LockProbe()
+    seen = {}
+
+    class Farfield:
+        tree = object()
+
+    class H:
+        farfield = Farfield()
+
+    class Prob:
+        def __init__(self, *a, **kw):
+            self.H, self.cm, self.rhs = H(), None, None
+
+    class CmT:
+        def dot_dev(self, x):
+            return x
+
+    class Op:
+        def __init__(self, H, cm):
+            self.cmT = CmT()
+
+        def enable_graphs(self, precs, stream=None):
+            seen['held'] = lock.held()
+            seen['precs'] = precs
+
+    monkeypatch.setattr(serve, 'TractionSlipProblem', Prob)
+    monkeypatch.setattr(serve, 'ConstrainedOperator', Op)
+    monkeypatch.setattr(serve, 'make_state', lambda *a, **kw: object())
+    serve._build({}, 'tree', 1e-8, 100, 1000, graphs=True, capture_stream=None,
+                 capture_lock=lock)
+    assert seen['held'] is True
+    assert not lock.held()
+
+
+def test_estimate_bytes_grows_with_size_and_tree():
+    a = serve.estimate_bytes(1000, 1500)
+    b = serve.estimate_bytes(4000, 1500)
+    assert 0 < a < b
+    assert serve.estimate_bytes(4000, 1500, tree=True) > b
