@@ -70,11 +70,11 @@ constexpr int n_acc() {
 
 // 3x3 kernel tensor (tectosaur/kernels/__init__.py tensor_code), m = obs
 // normal, n = src normal, D = y - x.
+// invr = 1/r, or 0 to drop the point (every entry is then 0).
 template <int KID>
-__device__ __forceinline__ void tensor(const double D[3], double r2, const double m[3],
-                                       const double n[3], const ElasticConsts &c,
-                                       double K[9]) {
-    const double invr = rsqrt(r2);
+__device__ __forceinline__ void tensor_invr(const double D[3], double invr, const double m[3],
+                                            const double n[3], const ElasticConsts &c,
+                                            double K[9]) {
     const double invr2 = invr * invr;
     const double invr3 = invr2 * invr;
     if constexpr (KID == TCT_U) {
@@ -127,6 +127,13 @@ __device__ __forceinline__ void tensor(const double D[3], double r2, const doubl
             for (int j = 0; j < 3; ++j)
                 K[i * 3 + j] = n[i] * NT[j] + m[i] * MT[j] + Dr[i] * DT[j] + (i == j ? ST : 0.0);
     }
+}
+
+template <int KID>
+__device__ __forceinline__ void tensor(const double D[3], double r2, const double m[3],
+                                       const double n[3], const ElasticConsts &c,
+                                       double K[9]) {
+    tensor_invr<KID>(D, rsqrt(r2), m, n, c, K);
 }
 
 // Symmetric index of (p, t) in 6-vector order xx, xy, xz, yy, yz, zz.

@@ -31,7 +31,6 @@ namespace tct {
 // written in edge form: D = b + sx se0 + sy se1 + ox oe0 + oy oe1 with
 // b = s_v0 - o_v0, se = s_v{1,2} - s_v0, oe = -(o_v{1,2} - o_v0): four FMAs
 // per component and quadrature point instead of two basis-weighted sums.
-// Used where no quadrature point can coincide (check0 off: near pairs).
 struct PairGeom {
     double b[3], se[2][3], oe[2][3];
 };
@@ -47,40 +46,42 @@ __device__ __forceinline__ void pair_geom(const Tri &ot, const Tri &st, PairGeom
     }
 }
 
+// One rule point of one pair.  CHECK0 skips the points at r = 0 (the "Z"
+// kernels, assemble.cl:24-47, and the adjacent kernels, :49-103, which test
+// r2 == 0 on the basis-weighted positions).  The edge form leaves a
+// rounding residue where the basis-weighted positions coincide exactly, so
+// the skip is decided from the rule instead: a pair whose two local
+// triangles are the same triangle in the same frame (`same`) has exactly
+// coincident points where the obs and src parameters are equal, and those
+// are the only exact zeros of the rules used here (every other point is
+// at least ~1e-4 triangle sizes apart: checked on the host for the
+// coincident rules and the product rules); r2 == 0 is skipped as well.  A
+// skipped point gets 1/r = 0, which zeroes every term it feeds.
 template <int KID, bool CHECK0>
 __device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, const PairGeom &g,
                                                  const double *q, double w,
-                                                 const ElasticConsts &c, double *acc) {
+                                                 const ElasticConsts &c, double *acc,
+                                                 bool same) {
     const double ox = q[0], oy = q[1], sx = q[2], sy = q[3];
     double D[3];
-    if constexpr (CHECK0) {
-        // pairs whose rule can hit r = 0 exactly: the reference's
-        // basis-weighted sums, so coincident points give D = 0 bit for bit
-        // and are skipped (the edge form leaves a rounding residue there)
-        const double po[3] = {1.0 - ox - oy, ox, oy};
-        const double ps[3] = {1.0 - sx - sy, sx, sy};
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double x = po[0] * ot.v[0][d] + po[1] * ot.v[1][d] + po[2] * ot.v[2][d];
-            const double y = ps[0] * st.v[0][d] + ps[1] * st.v[1][d] + ps[2] * st.v[2][d];
-            D[d] = y - x;
-        }
-    } else {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double t = fma(sx, g.se[0][d], g.b[d]);
-            t = fma(sy, g.se[1][d], t);
-            t = fma(ox, g.oe[0][d], t);
-            D[d] = fma(oy, g.oe[1][d], t);
-        }
+    for (int d = 0; d < 3; ++d) {
+        double t = fma(sx, g.se[0][d], g.b[d]);
+        t = fma(sy, g.se[1][d], t);
+        t = fma(ox, g.oe[0][d], t);
+        D[d] = fma(oy, g.oe[1][d], t);
     }
     const double r2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
-    if (CHECK0 && r2 == 0.0) return;
+    double invr = rsqrt(r2);
+    if constexpr (CHECK0) {
+        const bool skip = (same && ox == sx && oy == sy) || r2 == 0.0;
+        invr = skip ? 0.0 : invr;
+    }
     if constexpr (KID <= TCT_H) {
         const double po[3] = {1.0 - ox - oy, ox, oy};
         const double ps[3] = {1.0 - sx - sy, sx, sy};
         double K[9];
-        tensor<KID>(D, r2, ot.n, st.n, c, K);
+        tensor_invr<KID>(D, invr, ot.n, st.n, c, K);
 #pragma unroll
         for (int bo = 0; bo < 3; ++bo)
 #pragma unroll
@@ -93,7 +94,6 @@ __device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, c
                         acc[bo * 27 + dob * 9 + bs * 3 + ds] += f * K[dob * 3 + ds];
             }
     } else if constexpr (KID == TCT_RH) {
-        const double invr = rsqrt(r2);
         const double wi = w * invr;
         const double wi3 = wi * invr * invr;
         acc[0] += wi;
@@ -106,7 +106,6 @@ __device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, c
         // RT (weights follow the obs basis, normal = src) or RA (src basis, obs normal).
         const double po[3] = {1.0 - ox - oy, ox, oy};
         const double ps[3] = {1.0 - sx - sy, sx, sy};
-        const double invr = rsqrt(r2);
         const double invr3 = invr * invr * invr;
         const double *nn = KID == TCT_RT ? st.n : ot.n;
         const double *pL = KID == TCT_RT ? po : ps;
@@ -217,11 +216,13 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
     load_tri(st, pts, tris, pp[1], sc, flip);
     PairGeom g;
     pair_geom(ot, st, g);
+    // the two local triangles are one triangle in one frame
+    const bool same = pp[0] == pp[1] && oc == sc && !flip;
 
     double acc[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
-    if constexpr (GL == 1 && !CHECK0) {
+    if constexpr (GL == 1) {
         // one pair per lane: every lane walks the same rule, whose points are
         // wave-uniform scalar loads.  Those return out of order, so a wait
         // for one waits for all: the next point is loaded while the current
@@ -234,19 +235,29 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
             for (int iq = 0; iq + 1 < nq; iq += 2) {
                 fence_point(qa[4], g.b, acc[NA - 1]);
                 load_point(qb, qx, qw, iq + 1);
-                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc);
+                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc, same);
                 fence_point(qb[4], g.b, acc[NA - 1]);
                 load_point(qa, qx, qw, iq + 2 < nq ? iq + 2 : nq - 1);
-                accumulate_point<KID, CHECK0>(ot, st, g, qb, qb[4], c, acc);
+                accumulate_point<KID, CHECK0>(ot, st, g, qb, qb[4], c, acc, same);
             }
             if (nq & 1) {   // the last point, loaded by the final clamp
                 fence_point(qa[4], g.b, acc[NA - 1]);
-                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc);
+                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc, same);
             }
         }
-    } else if (live) {
-        for (int iq = sub; iq < nq; iq += GL)
-            accumulate_point<KID, CHECK0>(ot, st, g, qx + 4 * (size_t)iq, qw[iq], c, acc);
+    } else if (live && sub < nq) {
+        // lanes of a group take every GL-th point (consecutive lanes read
+        // consecutive points: coalesced vector loads); each lane's next
+        // point is loaded while its current one computes
+        double qa[5], qb[5];
+        load_point(qa, qx, qw, sub);
+        for (int iq = sub; iq < nq; iq += GL) {
+            const int nx = iq + GL < nq ? iq + GL : iq;
+            load_point(qb, qx, qw, nx);
+            accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc, same);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) qa[k] = qb[k];
+        }
     }
     if constexpr (GL > 1) {
 #pragma unroll
