@@ -36,11 +36,43 @@ PARAMS = [3e10, 0.25]
 # moments (6 x 3) = 65.
 RH_NEAR_FLOPS_PER_QP = 65
 # HBM bytes per launch from hardware counters (rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE in separate runs, scaled x2 and x1 by a 1 GiB copy measured in
-# the same runs): profiles/far_p2p_H_two_surface_442k_pmc_traffic.md.
-TRAFFIC_FAR_P2P_H_F32 = 1.752e9
-TRAFFIC_BSR_ICOSPHERE_L5 = 1.326e9
+# WRITE_SIZE in separate runs, scaled by a 1 GiB copy measured in the same
+# runs), read from the profile summary, which records the launch shape and
+# a hash of the kernel sources it was measured on: a figure measured on
+# other sources or another launch shape is not reported.
 TRAFFIC_SOURCE = 'profiles/far_p2p_H_two_surface_442k_pmc_traffic.md'
+TRAFFIC_KERNEL_SOURCES = {
+    'far_p2p_f32_kernel<H>': ('tectosaur_amd/csrc/far_p2p.hip', 'tectosaur_amd/csrc/far_pair.h'),
+    'bsr9_kernel': ('tectosaur_amd/csrc/sparse.hip',)}
+
+
+def sources_hash(files):
+    import hashlib
+    h = hashlib.sha256()
+    for f in files:
+        with open(os.path.join(ROOT, f), 'rb') as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profiled_traffic(kernel, tile):
+    """HBM bytes per launch of ``kernel`` at launch shape ``tile`` from the
+    machine-readable line of the traffic profile, or (None, reason)."""
+    import re
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_SOURCE)) as f:
+            txt = f.read()
+    except OSError:
+        return None, 'no profile'
+    have = sources_hash(TRAFFIC_KERNEL_SOURCES[kernel])
+    for m in re.finditer(r'<!-- traffic (\{.*?\}) -->', txt):
+        e = json.loads(m.group(1))
+        if e.get('kernel') != kernel or list(e.get('tile', [])) != list(tile):
+            continue
+        if e.get('sources') != have:
+            return None, 'profiled on other kernel sources (%s, now %s)' % (e.get('sources'), have)
+        return e['bytes'], TRAFFIC_SOURCE
+    return None, 'no profile of this kernel at this launch shape'
 
 
 def cfg2_mesh():
@@ -49,34 +81,23 @@ def cfg2_mesh():
 
 
 def time_matvec(op, v_dev, prec, steps, warmup):
+    """Seconds per matvec (wall clock over ``steps`` matvecs) and the mean
+    duration of the N-body launch inside those same matvecs (HIP events on
+    the stream it runs on)."""
     import torch
     out = torch.empty(op.shape[0], dtype=torch.float64, device=op.device)
     for _ in range(warmup):
         op.dot_dev(v_dev, out, prec)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        op.dot_dev(v_dev, out, prec)
+    for e in evs:
+        op.dot_dev(v_dev, out, prec, nbody_events=e)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps, out
-
-
-def kernel_event_time(op, prec, reps):
-    """Average duration of the N-body kernel alone, HIP events on the stream
-    the kernel is launched on (torch's current stream)."""
-    import torch
-    e = op.engine
-    s = torch.cuda.current_stream()
-    t = []
-    for _ in range(reps):
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(s)
-        e.run(prec)
-        b.record(s)
-        b.synchronize()
-        t.append(a.elapsed_time(b) * 1e-3)
-    return float(np.mean(t))
+    dt = (time.perf_counter() - t0) / steps
+    kt = float(np.mean([a.elapsed_time(b) for a, b in evs])) * 1e-3
+    return dt, out, kt
 
 
 def stream_event_time(fn, reps):
@@ -146,6 +167,8 @@ def nearfield_bench(level=5, nq=8, thr=3.0, reps=20):
     mat.dot_dev(x, y)
     bt = stream_event_time(lambda: mat.dot_dev(x, y), reps)
     gbs = mat.algorithmic_bytes / bt / 1e9
+    bsr_traffic, bsr_src = (profiled_traffic('bsr9_kernel', [int(mat.nnzb)])
+                            if level == 5 and nq == 8 else (None, 'not profiled'))
     res = {'workload': 'icosphere level %d (%d tris), rules of order %d, near threshold %g, '
                        'RH3 near / H3 correction, FP64' % (level, n, nq, thr),
            'pair_counts': op.counts, 'build_s': t_build,
@@ -157,8 +180,7 @@ def nearfield_bench(level=5, nq=8, thr=3.0, reps=20):
                             PEAK_FP64_TFLOPS},
            'bsr_matvec': {'blocks': int(mat.nnzb), 'bytes': int(mat.algorithmic_bytes),
                           'ms': bt * 1e3, 'GB_per_s': gbs, 'frac_hbm_peak': gbs / PEAK_HBM_GBS,
-                          'traffic': TRAFFIC_BSR_ICOSPHERE_L5 if level == 5 and nq == 8 else None,
-                          'traffic_source': TRAFFIC_SOURCE}}
+                          'traffic': bsr_traffic, 'traffic_source': bsr_src}}
     print('# nearfield: %s' % json.dumps(res), flush=True)
     return res
 
@@ -275,10 +297,14 @@ def large_mesh():
 def large_solve_bench(mode='tree', tol=1e-8):
     """One traction -> slip solve (RH3 near and far, continuity and
     free-edge constraints) on the 655,360-triangle fault-plus-topography
-    mesh: FP32 treecode farfield in the Arnoldi cycles, FP64 direct
-    residuals; traction (1, 0.5, -0.3) + N(0, 0.1) noise (default_rng(0)).
-    Reports set-up and solve times, iterations, residual evaluations, the
-    FP32/FP64 direct and treecode matvec times and peak device memory."""
+    mesh: FP32 treecode farfield in the Arnoldi cycles (the inner operator;
+    the workload's own direct FP32 farfield costs ~3 s per matvec, so a
+    solve with it as the inner operator would take ~17 min, reported as an
+    estimate from the measured matvec and the solve's inner matvec count),
+    FP64 direct residuals; traction (1, 0.5, -0.3) + N(0, 0.1) noise
+    (default_rng(0)).  Reports set-up and solve times, iterations, residual
+    evaluations, the FP32/FP64 direct and treecode matvec times and peak
+    device memory."""
     import torch
     from tectosaur_amd.solver import solve_constrained
     from tectosaur_amd.traction_slip import (DEFAULT_TREE, TractionSlipProblem,
@@ -317,8 +343,17 @@ def large_solve_bench(mode='tree', tol=1e-8):
     u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=tol, mode=mode, callback=progress)
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
+    inner = {'tree': 'FP32 treecode (order %d, mac %g) + FP32 nearfield blocks' % (
+                 DEFAULT_TREE['order'], DEFAULT_TREE['mac']),
+             'mixed': 'FP32 direct N-body + FP64 nearfield blocks',
+             'f64': 'FP64 direct N-body (no inner operator)'}[mode]
     out = {'workload': 'traction -> slip, %d-tri fault + topography mesh, RH3 near/far, '
                        'tol %g, mode %s' % (n, tol, mode),
+           'inner_operator': inner,
+           'residual_operator': 'FP64 direct farfield + FP64 nearfield',
+           'direct_f32_inner_estimate_s': (mv['f32_farfield_ms'] * 1e-3 *
+                                           (res.n_matvec.get('tree', 0) +
+                                            res.n_matvec.get('f32', 0))),
            'n_tris': int(n), 'n_dofs_free': int(prob.n_dofs), 'host_prep_s': t_prep,
            'device_build_s': t_build, 'solve_s': t_solve, 'converged': bool(res.converged),
            'iterations': int(res.iterations), 'restarts': int(res.restarts),
@@ -385,27 +420,27 @@ def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows, seconds=8.0):
 
 
 def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
-                modes=('tree', 'f64')):
+                modes=('tree', 'mixed', 'f64')):
     """Served traction -> slip solves: one Poisson stream of fault requests
-    through the continuous-batching server (FP32 treecode farfield in the
-    Arnoldi cycles, and FP32 direct) with FP64 refinement, and through the
-    plain FP64 one-at-a-time path; latency = finish - arrival (queueing,
-    device setup and solve).  Every result's residual is the FP64 direct
-    operator's.  ``modes``: 'tree' and 'mixed' (batched), 'f64' (sequential,
-    the baseline; always run)."""
+    through the continuous-batching server and through the plain FP64
+    one-at-a-time path (always run, the baseline); latency = finish -
+    arrival (queueing, device setup and solve).  Every result's residual is
+    the FP64 direct operator's.  Batched ``modes`` separate the gains:
+    'f64' (batching alone: the plain FP64 operator, 32 slots), 'mixed'
+    (+ FP32 direct farfield in the Arnoldi cycles with FP64 refinement) and
+    'tree' (+ the FP32 treecode as the inner farfield)."""
     import torch
     from tectosaur_amd.serve import Server, summarize
     from tectosaur_amd.traction_slip import random_requests
     warm = random_requests(2, rate=0.0, seed=seed + 99, n_min=512, n_max=1024)
     for m in modes:
-        if m in ('tree', 'mixed'):
-            Server(mode=m, max_active=2).run(warm, realtime=False)
+        Server(mode=m, max_active=2).run(warm, realtime=False)
     Server(mode='f64', max_active=1, prep_workers=0, graphs=False).run(warm, realtime=False)
     reqs = random_requests(n_req, rate=rate, seed=seed, n_min=min_tris, n_max=max_tris)
     out = {'workload': '%d fault requests, Poisson %.2f/s, strike U(0,360) dip U(20,90) '
                        'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near and'
                        ' far, tol 1e-8' % (n_req, rate, min_tris, max_tris)}
-    runs = [('batched_' + m, m, max_active, 4) for m in ('tree', 'mixed') if m in modes]
+    runs = [('batched_' + m, m, max_active, 4) for m in ('tree', 'mixed', 'f64') if m in modes]
     runs.append(('sequential_f64', 'f64', 1, 0))
     for name, mode, ma, pw in runs:
         srv = Server(mode=mode, max_active=ma, prep_workers=pw, graphs=(ma > 1))
@@ -435,9 +470,9 @@ def main():
     ap.add_argument('--cpu-rows', type=int, default=512)
     ap.add_argument('--kernels', default='elasticU3,elasticT3,elasticA3,elasticH3')
     ap.add_argument('--serve-requests', type=int, default=256)
-    ap.add_argument('--serve-modes', default='tree',
-                    help="batched modes to run beside the sequential FP64 path: 'tree', "
-                         "'mixed' or 'tree,mixed'")
+    ap.add_argument('--serve-modes', default='tree,mixed,f64',
+                    help="batched modes to run beside the sequential FP64 path, from "
+                         "'tree', 'mixed', 'f64'")
     ap.add_argument('--serve-max-tris', type=int, default=32768)
     ap.add_argument('--serve-rate', type=float, default=100.0)
     ap.add_argument('--no-serve', action='store_true')
@@ -481,12 +516,12 @@ def main():
         outs = {}
         for prec in ('f32', 'f64'):
             steps = args.steps if (K == 'elasticH3' and prec == 'f32') else 1
-            dt, out = time_matvec(op, v_dev, prec, steps, args.warmup if steps > 1 else 1)
+            dt, out, k_dt = time_matvec(op, v_dev, prec, steps, args.warmup if steps > 1 else 1)
             outs[prec] = out.cpu().numpy()
+            tc = op.engine.tile_cfg[prec]
             res[prec] = {'pair_evals_per_s': pe / dt, 'ms_per_matvec': dt * 1e3,
-                         'tile': [op.engine.tile_cfg[prec].block,
-                                  op.engine.tile_cfg[prec].obs_per_lane,
-                                  op.engine.tile_cfg[prec].src_chunks]}
+                         'nbody_ms': k_dt * 1e3,
+                         'tile': [tc.block, tc.obs_per_lane, tc.src_chunks, tc.flags]}
             print('# %s %s: %.3f ms/matvec, %.3e pair-evals/s' % (K, prec, dt * 1e3, pe / dt),
                   flush=True)
         res['f32_vs_f64_rel_l2'] = float(np.linalg.norm(outs['f32'] - outs['f64']) /
@@ -496,7 +531,8 @@ def main():
             H_out = outs
             H_op = op
             H_dt = res['f32']['ms_per_matvec'] * 1e-3
-            kt = kernel_event_time(op, 'f32', 3)
+            # the N-body's own duration inside the timed matvecs
+            kt = res['f32']['nbody_ms'] * 1e-3
         else:
             del op
         torch.cuda.empty_cache()
@@ -504,6 +540,8 @@ def main():
     pe = H_op.pair_evals
     flops = 102.0 * pe
     achieved = flops / kt / 1e12
+    H_tile = per_kernel['elasticH3']['f32']['tile']
+    traffic, traffic_src = profiled_traffic('far_p2p_f32_kernel<H>', H_tile)
     line = {
         'metric': 'H-kernel pair-evals/s + served-solve p50/p99 vs plain FP64, at 1e-8 residual',
         'value': pe / H_dt,
@@ -524,10 +562,12 @@ def main():
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'valu', 'achieved': achieved, 'peak': PEAK_FP32_TFLOPS,
                      'unit': 'TFLOP/s', 'frac': achieved / PEAK_FP32_TFLOPS,
-                     'traffic': TRAFFIC_FAR_P2P_H_F32, 'traffic_source': TRAFFIC_SOURCE,
-                     'algorithmic_bytes': int(n * 3 * (24 + 48 + 24 * H_op.engine.tile_cfg['f32'].src_chunks)),
+                     'traffic': traffic, 'traffic_source': traffic_src,
+                     'algorithmic_bytes': int(n * 3 * (24 + 48 + 24 * H_tile[2])),
                      'kernel': 'far_p2p_f32_kernel<H>', 'kernel_ms': kt * 1e3,
-                     'flops_per_pair': 102},
+                     'kernel_timing': 'HIP events around the N-body launch inside the %d '
+                                      'timed matvecs' % args.steps,
+                     'tile': H_tile, 'flops_per_pair': 102},
         'kernels': per_kernel,
     }
     if calib is not None:
@@ -542,7 +582,7 @@ def main():
         line['pair_finding'] = pairfind_bench(pts, tris)
     if not args.no_serve:
         line['served'] = serve_bench(args.serve_requests, args.serve_max_tris, args.serve_rate,
-                                     modes=tuple(args.serve_modes.split(',')) + ('f64',))
+                                     modes=tuple(m for m in args.serve_modes.split(',') if m))
     if not args.no_cpu_baseline:
         rng = np.random.default_rng(1)
         rows = np.sort(rng.choice(n, size=min(args.cpu_rows, n), replace=False))

@@ -9,7 +9,7 @@ tectosaur/constraint_builders.py (``find_free_edges`` :74-91,
 elimination itself runs in C++ (host_constraints.cpp) through the C ABI.
 
 The GMRES operator of a constrained solve is cm^T A cm, and the solution is
-u = cm x + c_rhs (examples/solve.py:229-265 of the reference).
+u = cm x + c_rhs (examples/solve.py:25-61 of the reference).
 """
 import collections
 import ctypes

@@ -133,7 +133,7 @@ def interior_pairs_quad(K_name, pairs_list, quad, obs_pts, obs_ns, src_pts, src_
 class TriToPtDirectFarfieldOp:
     """out[3i + d] = sum over every source triangle and quadrature point of
     K(y_q - x_i) (J w phi . v), pairs at r2 == 0 skipped: the reference's
-    TriToPtDirectFarfieldOp (tectosaur/interior.py:145-197) as gather +
+    TriToPtDirectFarfieldOp (tectosaur/interior.py:139-185) as gather +
     N-body + point reduction.  ``dot_dev`` writes a device tensor
     [n_obs * 3] in the caller's point order."""
 

@@ -2,7 +2,7 @@
 
 The reference solves cm^T A cm x = cm^T (b - A c_rhs) with
 scipy.sparse.linalg.gmres on the host, one host<->device round trip per
-matvec (tectosaur/simple_solver.py:14-34, examples/solve.py:229-265).  Here
+matvec (tectosaur/simple_solver.py:14-34, examples/solve.py:25-61).  Here
 the Krylov vectors, the operator and the orthogonalisation stay in HBM:
 
   * ``ConstrainedOperator`` applies cm^T (A (cm x)) with device CSR matvecs
@@ -506,7 +506,7 @@ def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxi
 
 
 def iterative_solve(iop, constraints, rhs=None, tol=1e-8, mode='mixed', restart=1500):
-    """Host-array API of examples/solve.py:229-265: constraints -> cm, solve
+    """Host-array API of examples/solve.py:25-61: constraints -> cm, solve
     cm^T (rhs - A c_rhs) and return (cm x + c_rhs as numpy, GmresResult)."""
     from .constraints import build_constraint_matrix
     cm, c_rhs, _ = build_constraint_matrix(constraints, iop.shape[1])

@@ -162,10 +162,12 @@ class TriToTriDirectFarfieldOp:
     def pair_evals(self):
         return self.engine.pair_evals
 
-    def dot_dev(self, v_dev, out_dev=None, prec=None):
+    def dot_dev(self, v_dev, out_dev=None, prec=None, nbody_events=None):
         """Device matvec: v_dev float64 [n_src*9] -> float64 [n_obs*9].
         prec: 'f32' or 'f64' (direct N-body), or 'tree' (FP32 treecode when
-        one is attached, else the FP32 direct N-body)."""
+        one is attached, else the FP32 direct N-body).  nbody_events: an
+        optional pair of torch events recorded on the current stream just
+        before and after the N-body launch (its duration inside a matvec)."""
         prec = prec or self.prec
         if prec == 'tree':
             if self.tree is not None:
@@ -174,7 +176,11 @@ class TriToTriDirectFarfieldOp:
         if out_dev is None:
             out_dev = torch.empty(self.shape[0], dtype=torch.float64, device=self.device)
         self._gather(v_dev, prec)
+        if nbody_events is not None:
+            nbody_events[0].record()
         self.engine.run(prec)
+        if nbody_events is not None:
+            nbody_events[1].record()
         self._scatter(prec, out_dev)
         return out_dev
 
