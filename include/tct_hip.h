@@ -158,6 +158,33 @@ int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float *loc32,
                 double eps2, double group_extent, const tct_tile_cfg *cfg,
                 double *out_partial, void *stream);
 
+/* Several problems' FP32 N-bodies in one launch (no reference counterpart:
+ * the reference launches farfield_tris_to_tris{K} once per operator and
+ * matvec, tectosaur/ops/sparse_farfield_op.py:50-59; a server batches the
+ * inner matvecs of many solves with this).  Each segment is one
+ * tct_far_p2p call's FP32 arguments (layouts from tct_far_pack_obs /
+ * tct_far_pack_src_ex and a strength gather; flags bit 0 only); all
+ * segments share the kernel, the constants (params_host) and the launch
+ * shape (block, obs_per_lane).  A segment's out_partial holds exactly what
+ * tct_far_p2p with the same src_chunks and flags writes (bit for bit: the
+ * per-point summation order depends only on the chunks). */
+typedef struct {
+    int64_t n_obs_pad;
+    const float *loc32;
+    const double *obs_org;
+    int64_t n_src_pad;
+    const float *rec;
+    const double *src_org;
+    double eps2;
+    double group_extent;
+    int32_t src_chunks;
+    int32_t flags;
+    double *out_partial;
+} tct_far_segment;
+
+int tct_far_p2p_batched(int kernel, const double *params_host, int block, int obs_per_lane,
+                        int n_seg, const tct_far_segment *segs, void *stream);
+
 /* Sum of the chunk partials: out [n][3] f64 (point order of the packing). */
 int tct_far_reduce_pts(int64_t n, int64_t n_obs_pad, int chunks, const double *partial,
                        double *out, void *stream);

@@ -28,6 +28,16 @@ class TctError(RuntimeError):
     pass
 
 
+class FarSegment(ctypes.Structure):
+    """tct_far_segment: one problem of a batched N-body launch."""
+    _fields_ = [('n_obs_pad', ctypes.c_int64), ('loc32', ctypes.c_void_p),
+                ('obs_org', ctypes.c_void_p), ('n_src_pad', ctypes.c_int64),
+                ('rec', ctypes.c_void_p), ('src_org', ctypes.c_void_p),
+                ('eps2', ctypes.c_double), ('group_extent', ctypes.c_double),
+                ('src_chunks', ctypes.c_int32), ('flags', ctypes.c_int32),
+                ('out_partial', ctypes.c_void_p)]
+
+
 class TileCfg(ctypes.Structure):
     _fields_ = [('block', ctypes.c_int32), ('obs_per_lane', ctypes.c_int32),
                 ('src_chunks', ctypes.c_int32), ('flags', ctypes.c_int32)]
@@ -52,6 +62,7 @@ SIGNATURES = {
     'tct_far_gather_tris': [_int, _int, _i64, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'tct_far_p2p': [_int, _int, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp,
                     _vp],
+    'tct_far_p2p_batched': [_int, _vp, _int, _int, _int, _vp, _vp],
     'tct_far_reduce_pts': [_i64, _i64, _int, _vp, _vp, _vp],
     'tct_tree_p2m': [_int, _int, _int, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
     'tct_tree_m2m': [_int, _int, _int, _int, _int, _int, _int, _vp, _vp, _vp],

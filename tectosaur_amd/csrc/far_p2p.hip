@@ -22,6 +22,8 @@
 //  * The source list can be split into chunks along grid.y; each chunk
 //    writes its own partial sum and the reduce/scatter kernels add the
 //    chunks in a fixed order (deterministic results).
+#include <type_traits>
+
 #include "far_pair.h"
 
 namespace tct {
@@ -30,9 +32,14 @@ template <int KID>
 constexpr int used_words() {
     return KID == TCT_RH ? 21 : KID == TCT_RT ? 12 : KID == TCT_H ? 11 : KID == TCT_T ? 10 : 9;
 }
+// Source records are read through the constant address space: the kernel
+// never writes them, and a uniform load from there is a scalar load however
+// the pointer was obtained (from a kernel argument or a segment table).
+typedef const __attribute__((address_space(4))) float *rec_ptr;
+
 // U consecutive records (the first NW of their RW words) into registers.
 template <int U, int NW, int RW>
-__device__ __forceinline__ void load_recs(float (&q)[U][NW], const float *__restrict__ r) {
+__device__ __forceinline__ void load_recs(float (&q)[U][NW], rec_ptr r) {
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -78,19 +85,73 @@ __device__ __forceinline__ BlockChunk block_chunk(int xcd_chunks, int obs_blocks
 // wave's observation group origins holds no pair with r2 <= eps2, so its
 // pairs skip the test (the same values: the test passes them through).
 // The last tile (it holds the padding records) always keeps the test.
-template <int KID, int P>
+// Several problems' N-bodies in one launch use a segment table passed by
+// value: segment s owns linear workgroups [blk0, next blk0), laid out as the
+// plain order of one problem's launch (observation block fastest, then
+// chunk).  One kernel and precision for all segments; each has its own
+// layout, chunks, constants and r2 test.
+struct FarSeg {
+    const float *loc32;
+    const double *obs_org;
+    const float *rec;
+    const double *src_org;
+    double *partial;
+    double sep2;
+    KConst<float> k;
+    int n_obs_pad, n_tiles, chunks, obs_blocks, blk0;
+};
+constexpr int FAR_SEG_MAX = 24;
+struct FarSegTable {
+    int n;
+    FarSeg s[FAR_SEG_MAX];
+};
+struct NoSegs {
+    int n;
+};
+template <bool SEG>
+using SegArg = typename std::conditional<SEG, FarSegTable, NoSegs>::type;
+
+// SEG = false: one problem, the arguments as named, workgroups mapped by
+// block_chunk.  SEG = true: the arguments come from this workgroup's
+// segment of `segs` (the named ones are ignored).  The body is the kernel's
+// own, not a shared device function: inlined into a kernel, a function body
+// let the compiler move the next records' scalar loads above the wait for
+// the current ones (they are invariant loads), which undid the pipelining.
+template <int KID, int P, bool SEG>
 __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
     int n_obs_pad, const float *__restrict__ loc32, const double *__restrict__ obs_org,
     int n_tiles, const float *__restrict__ rec, const double *__restrict__ src_org,
     KConst<float> k, double sep2, double *__restrict__ partial, int xcd_chunks,
-    int obs_blocks) {
+    int obs_blocks, SegArg<SEG> segs) {
+    int ob, c, chunks;
+    if constexpr (SEG) {
+        const int L = (int)blockIdx.x;
+        int s = 0;
+        while (s + 1 < segs.n && L >= segs.s[s + 1].blk0) ++s;
+        const FarSeg &g = segs.s[s];
+        const int l = L - g.blk0;
+        ob = l % g.obs_blocks;
+        c = l / g.obs_blocks;
+        chunks = g.chunks;
+        n_obs_pad = g.n_obs_pad;
+        loc32 = g.loc32;
+        obs_org = g.obs_org;
+        n_tiles = g.n_tiles;
+        rec = g.rec;
+        src_org = g.src_org;
+        k = g.k;
+        sep2 = g.sep2;
+        partial = g.partial;
+    } else {
+        const BlockChunk bc = block_chunk(xcd_chunks, obs_blocks);
+        ob = bc.ob;
+        c = bc.c;
+        chunks = bc.chunks;
+    }
     constexpr int RW = rec_words<KID>();
     constexpr int NO = n_out<KID>();
     const int bd = blockDim.x;
-    const BlockChunk bc = block_chunk(xcd_chunks, obs_blocks);
-    const int base = bc.ob * bd * P;
-    const int chunks = bc.chunks;
-    const int c = bc.c;
+    const int base = ob * bd * P;
     const int t0 = (int)(((long)n_tiles * c) / chunks);
     const int t1 = (int)(((long)n_tiles * (c + 1)) / chunks);
 
@@ -126,7 +187,7 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
 #pragma unroll
             for (int a = 0; a < n_mterm_arr<KID>(); ++a) am[p][a] = 0.0f;
         }
-        const float *r = rec + (size_t)t * G * RW;
+        const rec_ptr r = (rec_ptr)(rec + (size_t)t * G * RW);
         bool close = t == n_tiles - 1;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -549,8 +610,9 @@ static void launch_f32(dim3 grid, int block, hipStream_t st, int n_obs_pad, cons
                        const double *obs_org, int n_tiles, const float *rec,
                        const double *src_org, KConst<float> k, double sep2, double *partial,
                        int xcd_chunks, int obs_blocks) {
-    hipLaunchKernelGGL((far_p2p_f32_kernel<KID, P>), grid, dim3(block), 0, st, n_obs_pad, loc32,
-                       obs_org, n_tiles, rec, src_org, k, sep2, partial, xcd_chunks, obs_blocks);
+    hipLaunchKernelGGL((far_p2p_f32_kernel<KID, P, false>), grid, dim3(block), 0, st, n_obs_pad,
+                       loc32, obs_org, n_tiles, rec, src_org, k, sep2, partial, xcd_chunks,
+                       obs_blocks, NoSegs{0});
 }
 
 template <int KID, int P>
@@ -562,9 +624,104 @@ static void launch_f64(dim3 grid, int block, hipStream_t st, int n_obs_pad, cons
 }
 
 
+// sep^2 beyond which a source tile holds no pair with r2 <= eps2: points of
+// groups whose origins are d apart are at least d - group_extent apart
+// (group_extent = E_obs + E_src); INFINITY keeps the test on every tile.
+static double tile_sep2(double eps2, double group_extent, int flags) {
+    if (eps2 > 0.0 && group_extent >= 0.0 && std::isfinite(group_extent) && !(flags & 1)) {
+        const double e = std::sqrt(eps2);
+        const double sep = group_extent * (1.0 + 1e-3) + 5.0 * e;
+        return sep * sep;
+    }
+    return INFINITY;
+}
+
+template <int KID, int P>
+static void launch_f32_batched(unsigned blocks, int block, hipStream_t st, const FarSegTable &t) {
+    hipLaunchKernelGGL((far_p2p_f32_kernel<KID, P, true>), dim3(blocks), dim3(block), 0, st, 0,
+                       (const float *)nullptr, (const double *)nullptr, 0, (const float *)nullptr,
+                       (const double *)nullptr, KConst<float>{}, 0.0, (double *)nullptr, 0, 1, t);
+}
+
 }  // namespace tct
 
 using namespace tct;
+
+extern "C" int tct_far_p2p_batched(int kernel, const double *params_host, int block,
+                                   int obs_per_lane, int n_seg, const tct_far_segment *segs,
+                                   void *stream) {
+    TCT_CHECK_ARG(kernel >= TCT_U && kernel <= TCT_RH, "far_p2p_batched: kernel id %d", kernel);
+    TCT_CHECK_ARG(block == 64 || block == 128 || block == 256, "far_p2p_batched: block %d",
+                  block);
+    TCT_CHECK_ARG(obs_per_lane == 1 || obs_per_lane == 2 || obs_per_lane == 4,
+                  "far_p2p_batched: obs_per_lane %d", obs_per_lane);
+    TCT_CHECK_ARG(n_seg >= 0 && (n_seg == 0 || (segs && params_host)),
+                  "far_p2p_batched: bad segment list");
+    hipStream_t st = as_stream(stream);
+    const KConst<float> kc0 = kernel_consts<float>(kernel, params_host, 0.0);
+    for (int s0 = 0; s0 < n_seg; s0 += FAR_SEG_MAX) {
+        FarSegTable t;
+        t.n = n_seg - s0 < FAR_SEG_MAX ? n_seg - s0 : FAR_SEG_MAX;
+        int64_t blocks = 0;
+        for (int i = 0; i < t.n; ++i) {
+            const tct_far_segment &a = segs[s0 + i];
+            const int si = s0 + i;
+            TCT_CHECK_ARG(a.n_obs_pad > 0 && a.n_obs_pad % G == 0 && a.n_src_pad > 0 &&
+                              a.n_src_pad % G == 0 && a.n_obs_pad < (1LL << 31) &&
+                              a.n_src_pad < (1LL << 31),
+                          "far_p2p_batched: segment %d sizes must be positive multiples of %d",
+                          si, G);
+            TCT_CHECK_ARG(a.loc32 && a.obs_org && a.rec && a.src_org && a.out_partial,
+                          "far_p2p_batched: segment %d null pointer", si);
+            TCT_CHECK_ARG(a.eps2 >= 0.0 && std::isfinite(a.eps2) && !std::isnan(a.group_extent),
+                          "far_p2p_batched: segment %d eps2 / group_extent", si);
+            TCT_CHECK_ARG((a.flags & ~1) == 0, "far_p2p_batched: segment %d flags 0x%x", si,
+                          (unsigned)a.flags);
+            const int n_tiles = (int)(a.n_src_pad / G);
+            TCT_CHECK_ARG(a.src_chunks >= 1 && a.src_chunks <= n_tiles,
+                          "far_p2p_batched: segment %d src_chunks %d not in [1, %d]", si,
+                          a.src_chunks, n_tiles);
+            const int64_t ob = ceil_div(a.n_obs_pad, (int64_t)block * obs_per_lane);
+            FarSeg &g = t.s[i];
+            g.loc32 = a.loc32;
+            g.obs_org = a.obs_org;
+            g.rec = a.rec;
+            g.src_org = a.src_org;
+            g.partial = a.out_partial;
+            g.sep2 = tile_sep2(a.eps2, a.group_extent, a.flags);
+            g.k = kc0;
+            g.k.eps2 = (float)a.eps2;
+            g.n_obs_pad = (int)a.n_obs_pad;
+            g.n_tiles = n_tiles;
+            g.chunks = a.src_chunks;
+            g.obs_blocks = (int)ob;
+            g.blk0 = (int)blocks;
+            blocks += ob * a.src_chunks;
+            TCT_CHECK_ARG(blocks < (1LL << 31), "far_p2p_batched: grid too large");
+        }
+        if (blocks == 0) continue;
+#define TCT_B(KID, P) launch_f32_batched<KID, P>((unsigned)blocks, block, st, t)
+#define TCT_BP(KID)                                     \
+    switch (obs_per_lane) {                             \
+    case 1: TCT_B(KID, 1); break;                       \
+    case 2: TCT_B(KID, 2); break;                       \
+    default: TCT_B(KID, 4); break;                      \
+    }
+        switch (kernel) {
+        case TCT_U: TCT_BP(TCT_U); break;
+        case TCT_T: TCT_BP(TCT_T); break;
+        case TCT_A: TCT_BP(TCT_A); break;
+        case TCT_H: TCT_BP(TCT_H); break;
+        case TCT_RT: TCT_BP(TCT_RT); break;
+        case TCT_RA: TCT_BP(TCT_RA); break;
+        default: TCT_BP(TCT_RH); break;
+        }
+#undef TCT_BP
+#undef TCT_B
+        TCT_CHECK_LAUNCH("far_p2p_batched");
+    }
+    return TCT_OK;
+}
 
 extern "C" int tct_far_pack_obs(int64_t n, const double *pts, const double *ns, int64_t n_pad,
                                 float *loc32, double *org, double *soa64, void *stream) {
@@ -715,12 +872,7 @@ extern "C" int tct_far_p2p(int kernel, int prec, int64_t n_obs_pad, const float 
     // runs without the test.  The bound uses only the caller's geometry, so
     // any eps2 is safe.  eps2 = 0, a negative group_extent or bit 0 of
     // cfg.flags keeps the test everywhere.
-    double sep2 = INFINITY;
-    if (eps2 > 0.0 && group_extent >= 0.0 && std::isfinite(group_extent) && !(c.flags & 1)) {
-        const double e = std::sqrt(eps2);
-        const double sep = group_extent * (1.0 + 1e-3) + 5.0 * e;
-        sep2 = sep * sep;
-    }
+    const double sep2 = tile_sep2(eps2, group_extent, c.flags);
 #define TCT_F32_CASE(KID, P)                                                                 \
     launch_f32<KID, P>(grid, c.block, st, no, loc32, obs_org, n_tiles, (const float *)rec,    \
                        src_org, kernel_consts<float>(KID, params_host, eps2), sep2, out_partial, \

@@ -10,6 +10,8 @@ consecutive points is spatially compact: the FP32 path stores coordinates
 relative to an FP64 origin per group, and its accuracy depends on the group
 extent.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -208,6 +210,31 @@ class FarfieldEngine:
     @property
     def pair_evals(self):
         return self.n_obs * self.n_src
+
+
+def run_batched(engines, block=256, obs_per_lane=1):
+    """One launch for the FP32 N-bodies of several engines (same kernel and
+    constants), each with its own layout and source chunks; every engine's
+    partial-sum buffer gets what its own ``run('f32')`` would write.
+    Returns the partial-sum buffers."""
+    if not engines:
+        return []
+    e0 = engines[0]
+    for e in engines[1:]:
+        if e.kid != e0.kid or not np.array_equal(e.params, e0.params):
+            raise N.TctError('batched N-body: engines differ in kernel or constants')
+    segs = (N.FarSegment * len(engines))()
+    outs = []
+    for i, e in enumerate(engines):
+        cfg = e.tile_cfg['f32']
+        out = e.partial('f32')
+        outs.append(out)
+        segs[i] = N.FarSegment(e.n_obs_pad, N.ptr(e.loc32), N.ptr(e.obs_org), e.n_src_pad,
+                               N.ptr(e.rec32), N.ptr(e.src_org), float(e.eps2),
+                               e.group_extent, cfg.src_chunks, cfg.flags & 1, N.ptr(out))
+    N.call('tct_far_p2p_batched', e0.kid, e0.params.ctypes.data, block, obs_per_lane,
+           len(engines), ctypes.addressof(segs), N.stream_ptr())
+    return outs
 
 
 def ctypes_ref(cfg):

@@ -134,7 +134,7 @@ def _prepare(req, orders, tree=None, omp_threads=None):
 
 
 def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None,
-           capture_lock=None):
+           capture_lock=None, batch_nbody=False):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
     prob = TractionSlipProblem(None, None, None, farfield_precisions=precisions, prepared=prep)
     op = ConstrainedOperator(prob.H, prob.cm)
@@ -146,11 +146,15 @@ def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None,
     state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
     if graphs:
         precs = MODES[mode][0:2] + MODES[mode][3:4]
-        if capture_lock is None:
-            op.enable_graphs(precs, stream=capture_stream)
-        else:
-            with capture_lock:
-                op.enable_graphs(precs, stream=capture_stream)
+        # with batched N-bodies, the inner (FP32 direct) applies are split
+        # graphs around the N-body; the other precisions keep whole graphs
+        split = batch_nbody and mode == 'mixed'
+        whole = tuple(p for p in precs if not (split and p == MODES[mode][0]))
+        with capture_lock or contextlib.nullcontext():
+            op.enable_graphs(whole, stream=capture_stream)
+            if split:
+                op.enable_split_graphs((MODES[mode][0],), stream=capture_stream)
+        state.batch_split = split
     return prob, op, state
 
 
@@ -169,7 +173,7 @@ def _finalize(a, now, keep_slip):
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
                  hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None,
-                 graphs=True, streams=4, switch_interval=1e-3, builders=1):
+                 graphs=True, streams=4, switch_interval=1e-3, builders=1, batch_nbody=False):
         # device queries from worker threads start from cached values
         from . import _native as N
         from .autotune import device_arch
@@ -193,6 +197,13 @@ class Server:
         self.switch_interval = switch_interval
         # capture each solve's operator applications into HIP graphs
         self.graphs = graphs
+        # the FP32 direct N-bodies of all solves in a tick's batch that use
+        # the direct inner farfield run as one segment-table launch
+        # (farfield.run_batched) between each solve's split graphs; those
+        # solves share one stream, the group stream
+        self.batch_nbody = bool(batch_nbody and graphs)
+        self.group_stream = torch.cuda.Stream() if self.batch_nbody else None
+        self.nbody_launches = 0   # batched N-body launches issued
         # batched servers build new solves on a stream of their own
         # (default priority) so admissions overlap the running solves
         # (``builders`` threads, one build stream each; builds upload through
@@ -232,7 +243,8 @@ class Server:
         st = self._builder_local.stream
         with torch.cuda.stream(st):
             prob, op, state = _build(prep, self.mode, self.tol, self.restart, self.maxiter,
-                                     self.graphs, self.capture_stream, self.capture_lock)
+                                     self.graphs, self.capture_stream, self.capture_lock,
+                                     self.batch_nbody)
             ev = torch.cuda.Event()
             ev.record()
         return prob, op, state, ev, st
@@ -242,7 +254,9 @@ class Server:
         rec.pair_evals = prob.pair_evals
         rec.setup_s = clock() - rec.admitted
         stream = None
-        if self.streams:
+        if state.batch_split:
+            stream = self.group_stream
+        elif self.streams:
             stream = self.streams[self._next_stream % len(self.streams)]
             self._next_stream += 1
         if ev is not None:
@@ -259,13 +273,28 @@ class Server:
         until consumed, after the event, so nothing they occupy is reused
         while the gather may still read it."""
         from . import _native as N
-        outs = []
+        from .farfield import run_batched
+        outs = [None] * len(batch)
         used = {}
-        for a in batch:
-            outs.append(a.state.issue())
+        group = []
+        for i, a in enumerate(batch):
             a.busy = True
             if a.stream is not None:
                 used[id(a.stream)] = a.stream
+            if a.state.needs_nbody():
+                group.append(i)
+            else:
+                outs[i] = a.state.issue()
+        if group:
+            # on the group stream: every member's pre graph, one N-body
+            # launch for all of them, every member's post graph + CGS2
+            with torch.cuda.stream(self.group_stream):
+                for i in group:
+                    batch[i].state.issue_pre()
+                run_batched([batch[i].op.nbody_engine for i in group])
+                self.nbody_launches += 1
+                for i in group:
+                    outs[i] = batch[i].state.issue_post()
         k = self._slot
         self._slot ^= 1
         src, ln = self._seg_src[k], self._seg_len[k]
@@ -325,10 +354,11 @@ class Server:
         self._events = [torch.cuda.Event() for _ in range(2)]
         # one join event per solve stream; each event is recorded once here
         # so its handle exists before the native gather re-records it
-        self._join_events = {id(st): torch.cuda.Event() for st in self.streams}
-        self._join_st = np.zeros(max(1, len(self.streams)), dtype=np.uint64)
-        self._join_ev = np.zeros(max(1, len(self.streams)), dtype=np.uint64)
-        for st in self.streams:
+        all_streams = self.streams + ([self.group_stream] if self.group_stream else [])
+        self._join_events = {id(st): torch.cuda.Event() for st in all_streams}
+        self._join_st = np.zeros(max(1, len(all_streams)), dtype=np.uint64)
+        self._join_ev = np.zeros(max(1, len(all_streams)), dtype=np.uint64)
+        for st in all_streams:
             self._join_events[id(st)].record(st)
         for e in self._events:
             e.record()
@@ -382,7 +412,8 @@ class Server:
                                          need))
                     else:
                         prob, op, state = _build(prep, self.mode, self.tol, self.restart,
-                                                 self.maxiter, self.graphs, self.capture_stream)
+                                                 self.maxiter, self.graphs, self.capture_stream,
+                                                 batch_nbody=self.batch_nbody)
                         ev = None
                         if self.streams:
                             ev = torch.cuda.Event()

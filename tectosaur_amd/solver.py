@@ -96,6 +96,68 @@ class ConstrainedOperator:
             cur.wait_stream(side)
             self._graphs[prec] = g
 
+    def enable_split_graphs(self, precs, stream=None):
+        """Capture each apply of ``precs`` in two graphs around the farfield
+        N-body, which a server then runs for many solves in one launch
+        (farfield.run_batched): 'pre' = cm x, the farfield gather; 'post' =
+        the farfield scatter, the nearfield product, cm^T.  The kernels and
+        buffers are those of the whole apply, so pre + N-body + post gives
+        its result bit for bit.  The farfield must be the direct N-body
+        (TriToTriDirectFarfieldOp)."""
+        ff = getattr(self.A, 'farfield', None)
+        if ff is None or not hasattr(ff, '_gather') or not hasattr(ff, 'engine'):
+            raise ValueError('split graphs need a direct farfield operator')
+        n = self.shape[0]
+        if self._gx is None:
+            self._gx = torch.zeros(n, dtype=torch.float64, device=self.device)
+            self._gy = torch.empty(n, dtype=torch.float64, device=self.device)
+        if not hasattr(self, '_split'):
+            self._split = {}
+        cur = torch.cuda.current_stream(self.device)
+        side = stream if stream is not None else torch.cuda.Stream(self.device)
+        for prec in precs:
+            if prec is None or prec in self._split or prec == 'tree':
+                continue
+            self.A.prepare(prec)
+            pair = []
+            for part in (self._split_pre, self._split_post):
+                side.wait_stream(cur)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.stream(side):
+                    g.capture_begin(capture_error_mode='thread_local')
+                    try:
+                        part(prec)
+                    finally:
+                        g.capture_end()
+                cur.wait_stream(side)
+                pair.append(g)
+            self._split[prec] = tuple(pair)
+
+    def _split_pre(self, prec):
+        self.cm.dot_dev(self._gx, self._u)
+        self.A.farfield._gather(self._u, prec)
+
+    def _split_post(self, prec):
+        ff = self.A.farfield
+        ff._scatter(prec, self._y)
+        self.A.nearfield.dot_dev(self._u, self._y, accumulate=True, prec=prec)
+        self.cmT.dot_dev(self._y, self._gy)
+
+    def has_split(self, prec):
+        return prec in getattr(self, '_split', {})
+
+    def split_pre_replay(self, prec):
+        """The 'pre' graph of ``prec`` on the current stream."""
+        self._split[prec][0].replay()
+
+    def split_post_exec(self, prec):
+        """The executable 'post' graph of ``prec`` (a hipGraphExec_t address)."""
+        return self._split[prec][1].raw_cuda_graph_exec()
+
+    @property
+    def nbody_engine(self):
+        return self.A.farfield.engine
+
     def has_graph(self, prec):
         return prec in self._graphs
 
@@ -215,6 +277,11 @@ class GmresState:
         # the stream all of this solve's device work goes to (a server sets
         # it); None: the current stream at each call
         self.stream = None
+        # a server that runs the farfield N-body of many solves in one launch
+        # sets this: Arnoldi steps then go pre graph -> (the server's batched
+        # N-body) -> post graph + CGS2 (issue_pre / issue_post)
+        self.batch_split = False
+        self._split = False
         if self._hip:
             from . import _native as N
             nd = ctypes.c_int64(0)
@@ -245,23 +312,48 @@ class GmresState:
         return (torch.cuda.stream(self.stream) if self.stream is not None
                 else contextlib.nullcontext())
 
+    def needs_nbody(self):
+        """True when the next step is an Arnoldi step on split graphs: the
+        caller runs issue_pre, the farfield N-body (batched with other
+        solves') and then issue_post in place of issue."""
+        return self.phase == 'arnoldi' and self._split and not self.done
+
+    def issue_pre(self):
+        """The part of a split Arnoldi step before the N-body."""
+        with self._ctx():
+            self._op.split_pre_replay(self.inner_prec)
+
+    def issue_post(self):
+        """The rest of a split Arnoldi step after the N-body: the post graph
+        and CGS2 (one native call); returns what ``issue`` returns."""
+        return self._arnoldi_native()
+
     def issue(self):
         """Enqueue the next step's device work (on ``stream`` when set, else
         the current stream); return the device tensor whose host value
         ``consume`` needs."""
+        if self.phase == 'arnoldi' and self._split:
+            with self._ctx():
+                self._op.split_pre_replay(self.inner_prec)
+                from .farfield import run_batched
+                run_batched([self._op.nbody_engine])
+            return self._arnoldi_native()
         if self.phase == 'arnoldi' and self._fast:
-            # graph launch + CGS2 in one native call; the next basis vector
-            # goes straight into V[j+1] and the graph input
-            from . import _native as N
-            j = self.j
-            sp = self.stream.cuda_stream if self.stream is not None else N.stream_ptr()
-            N.call('tct_arnoldi_step', self._gexec, j + 1, self.n, self.n, self._pV, self._pgy,
-                   self._ph, self._psc, self._nsc, self._pV + (j + 1) * self._row_bytes,
-                   self._pgx, sp)
-            self._op.n_matvec[self.inner_prec] += 1
-            return self._h[:j + 2]
+            return self._arnoldi_native()
         with self._ctx():
             return self._issue()
+
+    def _arnoldi_native(self):
+        # graph launch + CGS2 in one native call; the next basis vector goes
+        # straight into V[j+1] and the graph input
+        from . import _native as N
+        j = self.j
+        sp = self.stream.cuda_stream if self.stream is not None else N.stream_ptr()
+        N.call('tct_arnoldi_step', self._gexec, j + 1, self.n, self.n, self._pV, self._pgy,
+               self._ph, self._psc, self._nsc, self._pV + (j + 1) * self._row_bytes,
+               self._pgx, sp)
+        self._op.n_matvec[self.inner_prec] += 1
+        return self._h[:j + 2]
 
     def _issue(self):
         if self.phase == 'norm_b':
@@ -349,13 +441,17 @@ class GmresState:
                 return self._finish('maxiter')
             self.restarts += 1
             self.V[0].copy_(self.r).div_(beta)
-            self._fast = (self._hip and self._op is not None and
-                          self._op.has_graph(self.inner_prec))
+            self._split = (self._hip and self._op is not None and self.batch_split and
+                           self._op.has_split(self.inner_prec))
+            self._fast = self._split or (self._hip and self._op is not None and
+                                         self._op.has_graph(self.inner_prec))
             if self._fast:
                 op = self._op
                 op.gx.copy_(self.V[0])
-                # addresses of the native Arnoldi step
-                self._gexec = op.graph_exec(self.inner_prec)
+                # addresses of the native Arnoldi step (with split graphs,
+                # the step's graph is the 'post' one)
+                self._gexec = (op.split_post_exec(self.inner_prec) if self._split else
+                               op.graph_exec(self.inner_prec))
                 self._pV, self._pgx, self._pgy = (self.V.data_ptr(), op.gx.data_ptr(),
                                                   op.gy.data_ptr())
                 self._ph, self._psc = self._h.data_ptr(), self._scratch.data_ptr()

@@ -279,3 +279,38 @@ def test_tile_skip_is_safe_for_any_eps2(K):
         outs.append(op.dot(v))
     assert np.all(np.isfinite(outs[0]))
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize('K', ['elasticH3', 'elasticRH3', 'elasticU3'])
+def test_batched_nbody_equals_single_launches(K):
+    # problems of different sizes (and subsets) in one segment-table launch:
+    # each one's partial sums are bit for bit those of its own launch with
+    # the same source chunks, also when the table spans several launches
+    import torch
+    from tectosaur_amd import _native as N
+    from tectosaur_amd.farfield import run_batched
+    from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
+    ops = []
+    for k, nx in enumerate((5, 9, 17, 12, 7) * 6):
+        pts, tris = mesh.make_rect(nx, nx + 2, [[-1, 0, 1], [1, 0, 1], [1, 0.3, -1],
+                                                [-1, -0.2, -1]])
+        n = tris.shape[0]
+        obs = np.arange(n) if k % 3 else np.arange(n // 2, n)
+        op = TriToTriDirectFarfieldOp(1, K, [3e10, 0.25], pts, tris, np.float32, obs,
+                                      np.arange(n))
+        v = torch.as_tensor(np.random.default_rng(k).normal(size=op.shape[1]), device='cuda')
+        op._gather(v, 'f32')
+        ops.append(op)
+    single = []
+    for op in ops:
+        single.append(op.engine.run('f32').clone())
+    outs = run_batched([op.engine for op in ops])
+    torch.cuda.synchronize()
+    assert len(outs) == len(ops) > 24
+    for a, b in zip(single, outs):
+        assert torch.equal(a, b)
+    # a launch shape other than the engines' own gives the same bits
+    outs2 = [o.clone() for o in run_batched([op.engine for op in ops], block=128,
+                                            obs_per_lane=2)]
+    for a, b in zip(single, outs2):
+        assert torch.equal(a, b)

@@ -85,12 +85,27 @@ def test_build_captures_under_the_capture_lock(monkeypatch):
             seen['held'] = lock.held()
             seen['precs'] = precs
 
+        def enable_split_graphs(self, precs, stream=None):
+            seen['split_held'] = lock.held()
+            seen['split_precs'] = precs
+
+    class State:
+        batch_split = False
+
     monkeypatch.setattr(serve, 'TractionSlipProblem', Prob)
     monkeypatch.setattr(serve, 'ConstrainedOperator', Op)
-    monkeypatch.setattr(serve, 'make_state', lambda *a, **kw: object())
-    serve._build({}, 'tree', 1e-8, 100, 1000, graphs=True, capture_stream=None,
-                 capture_lock=lock)
-    assert seen['held'] is True
+    monkeypatch.setattr(serve, 'make_state', lambda *a, **kw: State())
+    _, _, st = serve._build({}, 'tree', 1e-8, 100, 1000, graphs=True, capture_stream=None,
+                            capture_lock=lock)
+    assert seen['held'] is True and 'split_held' not in seen and not st.batch_split
+    assert not lock.held()
+    # a direct-farfield solve of a batching server: the inner precision's
+    # apply is captured as split graphs, also under the lock
+    Farfield.tree = None
+    _, _, st = serve._build({}, 'tree', 1e-8, 100, 1000, graphs=True, capture_stream=None,
+                            capture_lock=lock, batch_nbody=True)
+    assert seen['split_held'] is True and seen['split_precs'] == ('f32',)
+    assert 'f32' not in seen['precs'] and st.batch_split
     assert not lock.held()
 
 

@@ -86,17 +86,32 @@ def test_mixed_matches_f64_solution():
     assert np.linalg.norm(u32 - u64) <= 1e-8 * np.linalg.norm(u64)
 
 
-def test_server_result_same_alone_and_in_mix():
-    from tectosaur_amd.serve import Server
+def _mix_requests(n=8, seed=3):
+    """n requests spanning 512 to 8,192 triangles (log-uniform), the first
+    and last pinned to the ends of the range."""
     from tectosaur_amd.traction_slip import random_requests
-    reqs = random_requests(5, rate=0.0, seed=3, n_min=256, n_max=900)
-    mix = Server(mode='mixed', max_active=5, restart=60, orders=ORD, keep_slip=True)
+    reqs = random_requests(n, rate=0.0, seed=seed, n_min=512, n_max=8192)
+    reqs[0].n_tris, reqs[-1].n_tris = 512, 8192
+    return reqs
+
+
+@pytest.mark.parametrize('batch_nbody', [False, True])
+def test_server_result_same_alone_and_in_mix(batch_nbody):
+    # batch_nbody: the solves' FP32 N-bodies run as one segment-table launch
+    # per tick; a solve alone runs the same path (a batch of one)
+    from tectosaur_amd.serve import Server
+    reqs = _mix_requests()
+    mix = Server(mode='mixed', max_active=8, restart=500, orders=ORD, keep_slip=True,
+                 batch_nbody=batch_nbody)
     recs_mix = mix.run(reqs, realtime=False)
-    assert mix.max_concurrent >= 2  # solves were in flight together
+    assert mix.max_concurrent >= 4  # solves were in flight together
+    if batch_nbody:
+        assert mix.nbody_launches > 0
     for req, rm in zip(reqs, recs_mix):
-        alone = Server(mode='mixed', max_active=1, restart=60, orders=ORD, keep_slip=True)
+        alone = Server(mode='mixed', max_active=1, restart=500, orders=ORD, keep_slip=True,
+                       batch_nbody=batch_nbody)
         ra = alone.run([req], realtime=False)[0]
-        assert rm.status == ra.status == 'converged'
+        assert rm.status == ra.status == 'converged' and rm.rel_residual <= 1e-8
         assert rm.iterations == ra.iterations
         # identical kernels on identical buffers: agreement to roundoff (bitwise in practice)
         assert np.abs(rm.slip - ra.slip).max() <= 1e-12 * np.abs(ra.slip).max()
@@ -105,15 +120,16 @@ def test_server_result_same_alone_and_in_mix():
 def test_tree_server_result_same_alone_and_in_mix():
     from tectosaur_amd.serve import Server
     from tectosaur_amd.traction_slip import random_requests
-    reqs = random_requests(4, rate=0.0, seed=8, n_min=400, n_max=1500)
-    tree = dict(order=4, mac=2.5, min_tris=0)
+    reqs = _mix_requests(seed=8)
+    # the treecode above 2,000 triangles, the FP32 direct farfield below
+    tree = dict(order=4, mac=2.5, min_tris=2000)
     # two builder threads (each with its own build stream; captures serialised)
-    mix = Server(mode='tree', max_active=4, restart=100, orders=ORD, keep_slip=True, tree=tree,
-                 builders=2)
+    mix = Server(mode='tree', max_active=8, restart=500, orders=ORD, keep_slip=True, tree=tree,
+                 builders=2, batch_nbody=True)
     recs_mix = mix.run(reqs, realtime=False)
     for req, rm in zip(reqs, recs_mix):
-        ra = Server(mode='tree', max_active=1, restart=100, orders=ORD, keep_slip=True,
-                    tree=tree).run([req], realtime=False)[0]
+        ra = Server(mode='tree', max_active=1, restart=500, orders=ORD, keep_slip=True,
+                    tree=tree, batch_nbody=True).run([req], realtime=False)[0]
         assert rm.status == ra.status == 'converged' and rm.rel_residual <= 1e-8
         assert rm.iterations == ra.iterations
         assert np.abs(rm.slip - ra.slip).max() <= 1e-12 * np.abs(ra.slip).max()
