@@ -25,9 +25,12 @@ Pinning (what fixes each function independently of this restatement):
       (assemble.cl:85-89) departs from that for nonzero clicks, and the
       oracle reproduces exactly the computed departure
       (tests/test_oracle.py::test_edge_adjacent_numbering_invariance).
-  pairs_integrate (RT/RA/RH)      refinement additivity and convergence in n;
-      parity unpinned against the reference's numbers (no fixture exists and
-      the reference cannot run here).
+  pairs_integrate (RT/RA/RH)      no reference fixture exists; pinned by
+      the mathematics instead: on two disjoint closed icospheres integration
+      by parts makes RT = T, cm^T RA = cm^T A and cm^T RH = cm^T H (to
+      <= 2e-9 at order 6, spectrally falling with order), so a wrong sign,
+      constant or curl index fails; plus refinement additivity and
+      convergence in n.
   farfield_tris                   equals the dense pair integral of the same
       rule summed against the input (two independent code paths), plus the
       golden two-triangle fixture through that identity.
