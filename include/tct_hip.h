@@ -379,6 +379,19 @@ int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w, double *h
 int tct_krylov_combine(int m, int64_t n, int64_t ld, const double *V, const double *c,
                        double alpha, double beta, double *out, void *stream);
 
+/* tct_cgs2 without the scratch argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The partial sums use
+ * a static device array of TCT_CGS2_NOALLOC_MAX_DOUBLES doubles (32 MiB,
+ * held by each device the library runs on until the process ends); a call
+ * whose tct_krylov_scratch(m, n) exceeds it returns TCT_EARG.  Nothing is
+ * allocated and nothing needs clearing; results are bit-identical to
+ * tct_cgs2.  Calls that overlap on one device (concurrent streams or
+ * threads) share the array and race: serialise them. */
+#define TCT_CGS2_NOALLOC_MAX_DOUBLES (1LL << 22)
+int tct_cgs2_noalloc(int m, int64_t n, int64_t ld, const double *V, double *w, double *h,
+                     double *vnext, double *vnext2, void *stream);
+
 /* Serving-loop plumbing (no reference counterpart: the reference's solves
  * are one at a time on the host).
  * tct_arnoldi_step: one Arnoldi step of a solve whose operator application

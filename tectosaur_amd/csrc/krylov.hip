@@ -8,8 +8,11 @@
 //
 // Every reduction has a fixed order (per-block partials over fixed point
 // ranges, then one pass over the blocks), so results do not depend on
-// timing, and no library workspace is involved: solves on different
-// streams share nothing.  HBM-bound: each pass reads V[:m] once.
+// timing, and tct_cgs2 uses no library workspace: solves on different
+// streams share nothing (tct_cgs2_noalloc instead uses one static array per
+// device, below).  HBM-bound: each pass reads V[:m] once.
+#include <mutex>
+
 #include "tct_common.h"
 
 namespace tct {
@@ -246,6 +249,67 @@ extern "C" int tct_cgs2(int m, int64_t n, int64_t ld, const double *V, double *w
                   m);
     cgs2_launch(m, n, ld, V, w, h, scratch, vnext, vnext2, as_stream(stream));
     TCT_CHECK_LAUNCH("cgs2");
+    return TCT_OK;
+}
+
+// ---- CGS2 without a workspace argument ------------------------------------
+// The partial sums live in a file-scope device array instead of a caller
+// buffer, for callers that may not allocate (code under graph capture,
+// serving loops).  Each device holds its own copy for the life of the
+// process; its address is looked up once per device with
+// hipGetSymbolAddress and only that address is passed to kernels (the
+// array's name is not a device address in host code).  Every kernel writes
+// the part of the array it later reads, so nothing needs clearing between
+// calls; calls that overlap on one device share the array and race.
+constexpr int64_t CGS2_NOALLOC_MAX_DOUBLES = TCT_CGS2_NOALLOC_MAX_DOUBLES;  // 32 MiB
+
+static __device__ __align__(256) double scratch[CGS2_NOALLOC_MAX_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+// This device's address of `scratch` (looked up on the first call per device).
+static int scratch_address(double **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return set_error(TCT_EHIP, "cgs2_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES)
+        return set_error(TCT_EARG, "cgs2_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(TCT_EHIP, "cgs2_noalloc: hipGetSymbolAddress: %s",
+                             hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return TCT_OK;
+}
+
+extern "C" int tct_cgs2_noalloc(int m, int64_t n, int64_t ld, const double *V, double *w,
+                                double *h, double *vnext, double *vnext2, void *stream) {
+    TCT_CHECK_ARG(m >= 1 && n >= 1 && ld >= n && m < (1 << 20),
+                  "cgs2_noalloc: m %d n %lld ld %lld", m, (long long)n, (long long)ld);
+    TCT_CHECK_ARG(V && w && h, "cgs2_noalloc: null pointer");
+    TCT_CHECK_ARG(sizeof(double) * (size_t)m <= 60 * 1024,
+                  "cgs2_noalloc: basis size %d too large for LDS", m);
+    const int nb = n_blocks(n);
+    const int64_t need = 2 * (int64_t)nb * m + 2 * m;
+    TCT_CHECK_ARG(need <= CGS2_NOALLOC_MAX_DOUBLES,
+                  "cgs2_noalloc: m %d, n %lld need %lld doubles of scratch, more than the "
+                  "static %lld",
+                  m, (long long)n, (long long)need, (long long)CGS2_NOALLOC_MAX_DOUBLES);
+    double *s = nullptr;
+    const int rc = scratch_address(&s);
+    if (rc != TCT_OK) return rc;
+    cgs2_launch(m, n, ld, V, w, h, s, vnext, vnext2, as_stream(stream));
+    TCT_CHECK_LAUNCH("cgs2_noalloc");
     return TCT_OK;
 }
 
