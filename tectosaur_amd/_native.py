@@ -83,6 +83,7 @@ SIGNATURES = {
     'tct_cgs2': [_int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     'tct_krylov_combine': [_int, _i64, _i64, _vp, _vp, _dbl, _dbl, _vp, _vp],
     'tct_cgs2_noalloc': [_int, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    'tct_calib_valu': [_int, _int, _int, _vp, _vp],
     'tct_arnoldi_step': [_vp, _int, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     'tct_tick_gather': [_int, _vp, _vp, _int, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     'tct_interior_pairs_sinh': [_int, _int, _vp, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _i64,

@@ -80,11 +80,15 @@ def lookup(K_name, prec, n_obs_pad, n_src_pad, device=None):
         n_tiles = n_src_pad // GROUP
         chunks = max(1, min(int(e['src_chunks']), n_tiles))
         flags = 2 if (e.get('xcd') and chunks % 8 == 0) else 0
+        if e.get('lds') and prec == 'f32':
+            flags |= 4
         return N.TileCfg(int(e['block']), int(e['obs_per_lane']), chunks, flags)
     return default_tile_cfg(n_obs_pad, n_src_pad, prec, device)
 
 
-def candidates(n_obs_pad, n_src_pad, n_cu=256):
+def candidates(n_obs_pad, n_src_pad, n_cu=256, lds=False):
+    """Launch shapes to time.  ``lds`` adds every shape again with the
+    source tiles staged in LDS (flags bit 2; FP32 only)."""
     from .farfield import GROUP
     n_tiles = n_src_pad // GROUP
     out = []
@@ -100,6 +104,8 @@ def candidates(n_obs_pad, n_src_pad, n_cu=256):
             for chunks in (8, 16, 24, 32):
                 if chunks <= n_tiles:
                     out.append((block, per_lane, chunks, 2))
+    if lds:
+        out += [(b, p, c, f | 4) for b, p, c, f in out]
     return out
 
 
@@ -121,7 +127,8 @@ def tune(engine, prec, reps=3, save=True, verbose=False):
     # 256), not the device's: the chunk count sets the summation order, so
     # a table tuned on any box gives the same results on every box
     best = None
-    for block, per_lane, chunks, flags in candidates(engine.n_obs_pad, engine.n_src_pad):
+    for block, per_lane, chunks, flags in candidates(engine.n_obs_pad, engine.n_src_pad,
+                                                     lds=(prec == 'f32')):
         cfg = N.TileCfg(block, per_lane, chunks, flags)
         _run_and_reduce(engine, prec, cfg)  # warm-up / allocation
         torch.cuda.synchronize()
@@ -143,7 +150,7 @@ def tune(engine, prec, reps=3, save=True, verbose=False):
             t[key] = {
                 'block': best[0].block, 'obs_per_lane': best[0].obs_per_lane,
                 'src_chunks': best[0].src_chunks, 'xcd': bool(best[0].flags & 2),
-                'seconds': best[1]}
+                'lds': bool(best[0].flags & 4), 'seconds': best[1]}
             try:
                 with open(table_path(), 'w') as f:
                     json.dump(t, f, indent=1, sort_keys=True)

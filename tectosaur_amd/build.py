@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libtct_hip.so')
 HIP_SOURCES = ['abi.hip', 'far_p2p.hip', 'far_tree.hip', 'far_tree_eval_u.hip',
                'far_tree_eval_rt.hip', 'far_tree_eval_ra.hip', 'far_tree_eval_rh.hip', 'pairs.hip',
-               'interior.hip', 'near_find.hip', 'krylov.hip', 'sparse.hip']
+               'interior.hip', 'near_find.hip', 'krylov.hip', 'sparse.hip', 'calib.hip']
 HOST_SOURCES = ['host_mesh.cpp', 'host_constraints.cpp']
 ARCH = 'gfx950'
 

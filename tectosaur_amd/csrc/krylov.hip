@@ -19,34 +19,59 @@ namespace tct {
 
 constexpr int KB = 256;  // threads per block
 
-// partial[b][i] = sum over block b's points of V[i][t] w[t], i < m, and
-// partial[b][m] = sum w[t]^2 when with_norm.  Grid-stride over t.
+constexpr int CB = 8;  // columns per pass of the dot products
+
+// The block sums of CB per-thread values, in a fixed order (xor butterfly in
+// each wavefront, then the wavefronts in order): column i0 + k of block b
+// goes to partial[(i0 + k) * nb + b] for i0 + k < cols.  Partial sums are
+// stored column by column, [cols][nb], so a column's nb partials are
+// contiguous for the column reduction.
+__device__ __forceinline__ void store_block_sums(double (&s)[CB], double (*red)[CB], int i0,
+                                                 int cols, int nb, double *__restrict__ partial) {
+#pragma unroll
+    for (int k = 0; k < CB; ++k)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off, 64);
+    if (threadIdx.x % 64 == 0)
+#pragma unroll
+        for (int k = 0; k < CB; ++k) red[threadIdx.x / 64][k] = s[k];
+    __syncthreads();
+    if (threadIdx.x < CB && i0 + (int)threadIdx.x < cols) {
+        double r = 0.0;
+        for (int wv = 0; wv < KB / 64; ++wv) r += red[wv][threadIdx.x];
+        partial[(size_t)(i0 + threadIdx.x) * nb + blockIdx.x] = r;
+    }
+    __syncthreads();
+}
+
+// partial[i][b] = sum over block b's points of V[i][t] w[t], i < m, and
+// partial[m][b] = sum w[t]^2 when with_norm; CB columns per pass over the
+// block's points (one barrier pair per pass).
 __global__ __launch_bounds__(KB) void multi_dot_kernel(int m, int64_t n, int64_t ld,
                                                        const double *__restrict__ V,
                                                        const double *__restrict__ w,
                                                        int with_norm,
                                                        double *__restrict__ partial) {
-    __shared__ double red[KB / 64];
+    __shared__ double red[KB / 64][CB];
     const int nb = gridDim.x;
     const int cols = m + (with_norm ? 1 : 0);
     // this block's contiguous range of points
     const int64_t per = (n + nb - 1) / nb;
     const int64_t t0 = (int64_t)blockIdx.x * per;
     const int64_t t1 = t0 + per < n ? t0 + per : n;
-    for (int i = 0; i < cols; ++i) {
-        const double *vi = i < m ? V + (size_t)i * ld : w;
-        double s = 0.0;
-        for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) s += vi[t] * w[t];
+    for (int i0 = 0; i0 < cols; i0 += CB) {
+        double s[CB];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = 0.0;
-            for (int k = 0; k < KB / 64; ++k) r += red[k];
-            partial[(size_t)blockIdx.x * cols + i] = r;
+        for (int k = 0; k < CB; ++k) s[k] = 0.0;
+        for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) {
+            const double wt = w[t];
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const int i = i0 + k;
+                if (i < cols) s[k] += (i < m ? V[(size_t)i * ld + t] : wt) * wt;
+            }
         }
-        __syncthreads();
+        store_block_sums(s, red, i0, cols, nb, partial);
     }
 }
 
@@ -62,49 +87,54 @@ __global__ __launch_bounds__(KB) void combine_kernel(int m, int64_t n, int64_t l
     out[t] = (beta == 0.0 ? 0.0 : beta * out[t]) + alpha * s;
 }
 
-// Column sums of the per-block partials, each in block order b = 0..nb-1:
-// c[i] = sum_b partial[b][i], hout[i] = c[i] (+ add[i]).  One thread per
-// column; used when there are many blocks (below).
+// Column sums of the per-block partials, one wavefront per column: c[i] =
+// sum_b partial[i][b] (each lane takes every 64th block in order, then an
+// xor butterfly: a fixed order), hout[i] = c[i] (+ add[i]).  Used when there
+// are many blocks (below).
 __global__ __launch_bounds__(KB) void reduce_cols_kernel(int m, int nb,
                                                          const double *__restrict__ partial,
                                                          const double *__restrict__ add,
                                                          double *__restrict__ c,
                                                          double *__restrict__ hout) {
-    const int i = blockIdx.x * KB + threadIdx.x;
-    if (i >= m) return;
+    const int i = blockIdx.x * (KB / 64) + (int)threadIdx.x / 64;
+    const int lane = (int)threadIdx.x % 64;
+    if (i >= m) return;   // the whole wavefront
+    const double *p = partial + (size_t)i * nb;
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += partial[(size_t)b * m + i];
-    c[i] = s;
-    if (hout) hout[i] = add ? s + add[i] : s;
+    for (int b = lane; b < nb; b += 64) s += p[b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) {
+        c[i] = s;
+        if (hout) hout[i] = add ? s + add[i] : s;
+    }
 }
 
 // The reduce, the update and the next sums in one launch, on the partition
 // of multi_dot_kernel (gridDim.x == nb blocks of contiguous points): the
-// column sums c[i] of partial_in[b][0..m) in block order -- summed by every
+// column sums c[i] of partial_in[i][0..nb) in block order -- summed by every
 // block itself, block 0 writing c (+ add) to hout, or, with many blocks,
-// read from c_in where reduce_cols_kernel left them (the same sums, so the
-// same bits, without nb re-reads of the nb x m partials); w[t] -= sum_i
-// V[i][t] c[i] on the block's points; then partial_out[b][i] = sum over
-// those points of V[i][t] w[t] (norm_only: the one column sum w[t]^2).
-// Each w[t] is updated by one sum in a fixed order and the sums follow
-// multi_dot_kernel's per-thread order, so the values do not depend on the
-// launch shape and match an update launch followed by multi_dot_kernel bit
-// for bit; each thread reads back only the w[t] it wrote.  partial_in and
-// partial_out must not overlap.
+// read from c_in where reduce_cols_kernel left them (without nb re-reads of
+// the m x nb partials); w[t] -= sum_i V[i][t] c[i] on the block's points;
+// then partial_out[i][b] = sum over those points of V[i][t] w[t] (norm_only:
+// the one column sum w[t]^2), as multi_dot_kernel forms them.  Each w[t] is
+// updated by one sum in a fixed order, so the values do not depend on the
+// launch shape; each thread reads back only the w[t] it wrote.  partial_in
+// and partial_out must not overlap.
 __global__ __launch_bounds__(KB) void reduce_update_dot_kernel(
     int m, int64_t n, int64_t ld, const double *__restrict__ V,
     const double *__restrict__ partial_in, const double *__restrict__ c_in,
     const double *__restrict__ add, double *__restrict__ hout, double *__restrict__ w,
     int norm_only, double *__restrict__ partial_out) {
     extern __shared__ double c[];
-    __shared__ double red[KB / 64];
+    __shared__ double red[KB / 64][CB];
     const int nb = gridDim.x;
     for (int i = threadIdx.x; i < m; i += KB) {
         if (c_in) {
             c[i] = c_in[i];
         } else {
             double s = 0.0;
-            for (int b = 0; b < nb; ++b) s += partial_in[(size_t)b * m + i];
+            for (int b = 0; b < nb; ++b) s += partial_in[(size_t)i * nb + b];
             c[i] = s;
             if (blockIdx.x == 0 && hout) hout[i] = add ? s + add[i] : s;
         }
@@ -119,20 +149,19 @@ __global__ __launch_bounds__(KB) void reduce_update_dot_kernel(
         w[t] = w[t] - s;
     }
     const int cols = norm_only ? 1 : m;
-    for (int i = 0; i < cols; ++i) {
-        const double *vi = norm_only ? w : V + (size_t)i * ld;
-        double s = 0.0;
-        for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) s += vi[t] * w[t];
+    for (int i0 = 0; i0 < cols; i0 += CB) {
+        double s[CB];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double r = 0.0;
-            for (int k = 0; k < KB / 64; ++k) r += red[k];
-            partial_out[(size_t)blockIdx.x * cols + i] = r;
+        for (int k = 0; k < CB; ++k) s[k] = 0.0;
+        for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) {
+            const double wt = w[t];
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const int i = i0 + k;
+                if (i < cols) s[k] += (norm_only ? wt : V[(size_t)i * ld + t]) * wt;
+            }
         }
-        __syncthreads();
+        store_block_sums(s, red, i0, cols, nb, partial_out);
     }
 }
 
@@ -189,17 +218,17 @@ inline int n_blocks(int64_t n) {
 // block of the update (nb^2 m re-reads).
 constexpr int NB_SEPARATE_REDUCE = 128;
 
-// The four (or six) launches of one CGS2 step; scratch = p1 [nb][m],
-// p2 [nb][m], h1 [m], c [m].
+// The four (or six) launches of one CGS2 step; scratch = p1 [m][nb],
+// p2 [m][nb], h1 [m], c [m].
 inline void cgs2_launch(int m, int64_t n, int64_t ld, const double *V, double *w, double *h,
                         double *scratch, double *vnext, double *vnext2, hipStream_t st) {
     const int nb = n_blocks(n);
-    double *p1 = scratch;                     // [nb][m], later the norm partials
-    double *p2 = scratch + (size_t)nb * m;    // [nb][m]
+    double *p1 = scratch;                     // [m][nb], later the norm partials
+    double *p2 = scratch + (size_t)nb * m;    // [m][nb]
     double *h1 = scratch + 2 * (size_t)nb * m;
     double *cs = h1 + m;                      // column sums (separate reduce)
     const bool sep = nb >= NB_SEPARATE_REDUCE;
-    const unsigned gr = (unsigned)ceil_div((int64_t)m, (int64_t)KB);
+    const unsigned gr = (unsigned)ceil_div((int64_t)m, (int64_t)(KB / 64));
     // normalisation launch: enough blocks to cover the points, at most 1024
     const int64_t gpl = ceil_div(n, (int64_t)KB);
     const unsigned gp = (unsigned)(gpl < 1024 ? gpl : 1024);

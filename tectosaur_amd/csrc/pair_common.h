@@ -56,9 +56,9 @@ __device__ __forceinline__ void load_tri(Tri &t, const double *__restrict__ pts,
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
         const double e1 = v[1][s] - v[0][s], e2 = v[2][s] - v[0][s];
-        t.curl[0][s] = (-e2 + e1) / t.jac;
-        t.curl[1][s] = e2 / t.jac;
-        t.curl[2][s] = -e1 / t.jac;
+        t.curl[0][s] = (-e2 + e1) * inv;
+        t.curl[1][s] = e2 * inv;
+        t.curl[2][s] = -e1 * inv;
     }
 }
 

@@ -46,19 +46,39 @@ __device__ __forceinline__ void pair_geom(const Tri &ot, const Tri &st, PairGeom
     }
 }
 
-// One rule point of one pair.  CHECK0 skips the points at r = 0 (the "Z"
-// kernels, assemble.cl:24-47, and the adjacent kernels, :49-103, which test
-// r2 == 0 on the basis-weighted positions).  The edge form leaves a
-// rounding residue where the basis-weighted positions coincide exactly, so
-// the skip is decided from the rule instead: a pair whose two local
-// triangles are the same triangle in the same frame (`same`) has exactly
-// coincident points where the obs and src parameters are equal, and those
-// are the only exact zeros of the rules used here (every other point is
-// at least ~1e-4 triangle sizes apart: checked on the host for the
-// coincident rules and the product rules); r2 == 0 is skipped as well.  A
-// skipped point gets 1/r = 0, which zeroes every term it feeds.
-template <int KID, bool CHECK0>
-__device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, const PairGeom &g,
+// 1/sqrt(x) for finite x > 0: the hardware estimate refined by one
+// third-order Newton step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2, which
+// takes the estimate's error to below double rounding.  No special-value
+// handling: x = 0 gives a non-finite value, which the r = 0 test replaces.
+__device__ __forceinline__ double rsq_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y), y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
+// The r = 0 test of a pair class (CK): CK_NONE for pairs that never touch;
+// CK_FULL skips r2 == 0 and the points whose obs and src parameters are
+// equal on pairs that are one triangle in one frame; CK_R2 skips r2 == 0
+// only, for rules with no point of equal obs and src parameters.
+constexpr int CK_NONE = 0, CK_FULL = 1, CK_R2 = 2;
+
+// Normals a kernel body reads: obs (m) and src (n).
+struct PairNormals {
+    double m[3], n[3];
+};
+
+// One rule point of one pair.  The r = 0 test skips the points at r = 0
+// (the "Z" kernels, assemble.cl:24-47, and the adjacent kernels, :49-103,
+// which test r2 == 0 on the basis-weighted positions).  The edge form
+// leaves a rounding residue where the basis-weighted positions coincide
+// exactly, so the skip is decided from the rule instead: a pair whose two
+// local triangles are the same triangle in the same frame (`same`) has
+// exactly coincident points where the obs and src parameters are equal,
+// and those are the only exact zeros of a rule (the rules used here have
+// none: checked on the host, which then asks for CK_R2); r2 == 0 is skipped
+// as well.  A skipped point gets 1/r = 0, which zeroes every term it feeds.
+template <int KID, int CK>
+__device__ __forceinline__ void accumulate_point(const PairNormals &nm, const PairGeom &g,
                                                  const double *q, double w,
                                                  const ElasticConsts &c, double *acc,
                                                  bool same) {
@@ -72,16 +92,18 @@ __device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, c
         D[d] = fma(oy, g.oe[1][d], t);
     }
     const double r2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
-    double invr = rsqrt(r2);
-    if constexpr (CHECK0) {
+    double invr = rsq_pos(r2);
+    if constexpr (CK == CK_FULL) {
         const bool skip = (same && ox == sx && oy == sy) || r2 == 0.0;
         invr = skip ? 0.0 : invr;
+    } else if constexpr (CK == CK_R2) {
+        invr = r2 == 0.0 ? 0.0 : invr;
     }
     if constexpr (KID <= TCT_H) {
         const double po[3] = {1.0 - ox - oy, ox, oy};
         const double ps[3] = {1.0 - sx - sy, sx, sy};
         double K[9];
-        tensor_invr<KID>(D, invr, ot.n, st.n, c, K);
+        tensor_invr<KID>(D, invr, nm.m, nm.n, c, K);
 #pragma unroll
         for (int bo = 0; bo < 3; ++bo)
 #pragma unroll
@@ -107,7 +129,7 @@ __device__ __forceinline__ void accumulate_point(const Tri &ot, const Tri &st, c
         const double po[3] = {1.0 - ox - oy, ox, oy};
         const double ps[3] = {1.0 - sx - sy, sx, sy};
         const double invr3 = invr * invr * invr;
-        const double *nn = KID == TCT_RT ? st.n : ot.n;
+        const double *nn = KID == TCT_RT ? nm.n : nm.m;
         const double *pL = KID == TCT_RT ? po : ps;
         const double Q1 = c.CsRT1 * invr;
         const double Q2 = c.CsRT0 * invr3;
@@ -180,6 +202,12 @@ __device__ __forceinline__ void load_point(double (&q)[5], const double *__restr
     for (int k = 0; k < 4; ++k) q[k] = qx[4 * (size_t)iq + k];
     q[4] = qw[iq];
 }
+// The rule point at pq (4 parameters) and pw (weight).
+__device__ __forceinline__ void load_point_at(double (&q)[5], const double *pq, const double *pw) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = pq[k];
+    q[4] = pw[0];
+}
 // An empty statement that reads the last loaded word of the point (so the
 // scalar loads are waited for here), keeps memory accesses on their side of
 // it, and passes one accumulator and the pair's base offset b through: the
@@ -189,7 +217,7 @@ __device__ __forceinline__ void fence_point(double last, double (&b)[3], double 
     asm volatile("" : "+v"(b[0]), "+v"(b[1]), "+v"(b[2]), "+v"(dep) : "s"(last) : "memory");
 }
 
-template <int KID, int GL, bool CHECK0>
+template <int KID, int GL, int CK>
 __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__restrict__ qx,
                                                     const double *__restrict__ qw,
                                                     const double *__restrict__ pts,
@@ -211,11 +239,22 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
         sc = pp[3];
         flip = pp[4] != 0;
     }
-    Tri ot, st;
-    load_tri(ot, pts, tris, pp[0], oc, 0);
-    load_tri(st, pts, tris, pp[1], sc, flip);
+    // The rule loop needs only the pair geometry and the normals; the whole
+    // triangle records (curls, Jacobians) are formed again after it, so
+    // they hold no registers during the loop.
     PairGeom g;
-    pair_geom(ot, st, g);
+    PairNormals nm;
+    {
+        Tri ot, st;
+        load_tri(ot, pts, tris, pp[0], oc, 0);
+        load_tri(st, pts, tris, pp[1], sc, flip);
+        pair_geom(ot, st, g);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            nm.m[d] = ot.n[d];
+            nm.n[d] = st.n[d];
+        }
+    }
     // the two local triangles are one triangle in one frame
     const bool same = pp[0] == pp[1] && oc == sc && !flip;
 
@@ -235,28 +274,41 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
             for (int iq = 0; iq + 1 < nq; iq += 2) {
                 fence_point(qa[4], g.b, acc[NA - 1]);
                 load_point(qb, qx, qw, iq + 1);
-                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc, same);
+                accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
                 fence_point(qb[4], g.b, acc[NA - 1]);
                 load_point(qa, qx, qw, iq + 2 < nq ? iq + 2 : nq - 1);
-                accumulate_point<KID, CHECK0>(ot, st, g, qb, qb[4], c, acc, same);
+                accumulate_point<KID, CK>(nm, g, qb, qb[4], c, acc, same);
             }
             if (nq & 1) {   // the last point, loaded by the final clamp
                 fence_point(qa[4], g.b, acc[NA - 1]);
-                accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc, same);
+                accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
             }
         }
-    } else if (live && sub < nq) {
+    } else if (live) {
         // lanes of a group take every GL-th point (consecutive lanes read
-        // consecutive points: coalesced vector loads); each lane's next
-        // point is loaded while its current one computes
-        double qa[5], qb[5];
-        load_point(qa, qx, qw, sub);
-        for (int iq = sub; iq < nq; iq += GL) {
-            const int nx = iq + GL < nq ? iq + GL : iq;
-            load_point(qb, qx, qw, nx);
-            accumulate_point<KID, CHECK0>(ot, st, g, qa, qa[4], c, acc, same);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) qa[k] = qb[k];
+        // consecutive points: coalesced vector loads).  The first nfull
+        // double strides (2 GL points) are taken by every lane with one
+        // trip count and no index clamps, two points in flight in two
+        // register sets; each lane's at most two remaining points follow.
+        const int nfull = nq / (2 * GL);
+        const double *pq = qx + 4 * (size_t)sub;
+        const double *pw = qw + sub;
+        if (nfull > 0) {
+            double qa[5], qb[5];
+            load_point_at(qa, pq, pw);
+            for (int k = 0; k < nfull; ++k) {
+                load_point_at(qb, pq + 4 * GL, pw + GL);
+                accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
+                pq += 8 * GL;
+                pw += 2 * GL;
+                if (k + 1 < nfull) load_point_at(qa, pq, pw);
+                accumulate_point<KID, CK>(nm, g, qb, qb[4], c, acc, same);
+            }
+        }
+        for (int iq = 2 * GL * nfull + sub; iq < nq; iq += GL) {
+            double qa[5];
+            load_point(qa, qx, qw, iq);
+            accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
         }
     }
     if constexpr (GL > 1) {
@@ -269,6 +321,14 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
         }
     }
     if (!live || sub != 0) return;
+    // the triangle records again, from indices the compiler cannot match
+    // with the first loads (it would otherwise keep those records live
+    // through the rule loop)
+    int oi = pp[0], si = pp[1];
+    asm volatile("" : "+v"(oi), "+v"(si));
+    Tri ot, st;
+    load_tri(ot, pts, tris, oi, oc, 0);
+    load_tri(st, pts, tris, si, sc, flip);
 
     // De-rotation (assemble.cl:77-101): out[b1] = res[od[b1]] with
     // od[i] = (i - clicks) mod 3; for the source, entries 0 and 1 of the
@@ -316,12 +376,15 @@ static void launch_pairs(hipStream_t st, int nq, const double *qx, const double 
                          const ElasticConsts &c, double *out) {
     const int64_t threads = (int64_t)n_pairs * GL;
     const unsigned nb = (unsigned)ceil_div(threads, 256);
-    if (check0)
-        hipLaunchKernelGGL((pairs_kernel<KID, GL, true>), dim3(nb), dim3(256), 0, st, nq, qx, qw,
-                           pts, tris, n_pairs, pairs, pair_width, c, out);
+    if (check0 == CK_FULL)
+        hipLaunchKernelGGL((pairs_kernel<KID, GL, CK_FULL>), dim3(nb), dim3(256), 0, st, nq, qx,
+                           qw, pts, tris, n_pairs, pairs, pair_width, c, out);
+    else if (check0 == CK_R2)
+        hipLaunchKernelGGL((pairs_kernel<KID, GL, CK_R2>), dim3(nb), dim3(256), 0, st, nq, qx,
+                           qw, pts, tris, n_pairs, pairs, pair_width, c, out);
     else
-        hipLaunchKernelGGL((pairs_kernel<KID, GL, false>), dim3(nb), dim3(256), 0, st, nq, qx, qw,
-                           pts, tris, n_pairs, pairs, pair_width, c, out);
+        hipLaunchKernelGGL((pairs_kernel<KID, GL, CK_NONE>), dim3(nb), dim3(256), 0, st, nq, qx,
+                           qw, pts, tris, n_pairs, pairs, pair_width, c, out);
 }
 
 template <int KID>
@@ -351,6 +414,8 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
     TCT_CHECK_ARG(n_pairs >= 0 && n_pairs < (1LL << 26), "pairs: n_pairs %lld (max 2^26 per call)",
                   (long long)n_pairs);
     TCT_CHECK_ARG(pair_width == 2 || pair_width == 5, "pairs: pair_width %d", pair_width);
+    TCT_CHECK_ARG(check0 == CK_NONE || check0 == CK_FULL || check0 == CK_R2, "pairs: check0 %d",
+                  check0);
     TCT_CHECK_ARG(qx && qw && pts && tris && params_host && out && (pairs || n_pairs == 0),
                   "pairs: null pointer");
     if (n_pairs == 0) return TCT_OK;

@@ -27,6 +27,35 @@ _RULES = {}
 _RULES_LOCK = threading.Lock()
 
 
+class DeviceRule(tuple):
+    """A 4-D rule on the device, ``(qx [nq, 4], qw [nq])``.
+    ``has_equal_params`` says whether some point has equal observation and
+    source parameters: on a pair that is one triangle in one frame those
+    points are at r = 0, and the kernel's r = 0 test must look for them; a
+    rule without such points needs only the r2 == 0 test."""
+
+    def __new__(cls, qx, qw, has_equal_params):
+        t = super().__new__(cls, (qx, qw))
+        t.has_equal_params = bool(has_equal_params)
+        return t
+
+
+def rule_has_equal_params(qx):
+    """Whether some point of a 4-D rule (qx [nq, 4]: obs xhat, obs yhat,
+    src xhat, src yhat) has equal observation and source parameters."""
+    qx = np.asarray(qx, dtype=np.float64)
+    return bool(qx.size) and bool(np.any((qx[:, 0] == qx[:, 2]) & (qx[:, 1] == qx[:, 3])))
+
+
+def r0_test_mode(q, check0):
+    """The check0 argument of tct_pairs_integrate: 0 without the r = 0
+    test, 1 for the full test, 2 for r2 == 0 alone (rules known to have no
+    point of equal obs and src parameters)."""
+    if not check0:
+        return 0
+    return 1 if getattr(q, 'has_equal_params', True) else 2
+
+
 class PairsIntegrator:
     def __init__(self, kernel, params, float_type, nq_far, nq_near, pts, tris):
         self.kernel = kernel
@@ -44,8 +73,9 @@ class PairsIntegrator:
         self.far_q = self._rule(('far4d', nq_far), lambda: gauss4d_tri(nq_far, nq_far))
 
     def quad_to_gpu(self, q):
-        return (N.upload(q[0], torch.float64, self.device),
-                N.upload(q[1], torch.float64, self.device))
+        qx = np.asarray(q[0], dtype=np.float64)
+        return DeviceRule(N.upload(qx, torch.float64, self.device),
+                          N.upload(q[1], torch.float64, self.device), rule_has_equal_params(qx))
 
     def _rule(self, key, make):
         if key not in self._rules:
@@ -72,7 +102,7 @@ class PairsIntegrator:
             e = min(n, s + CALL_SIZE)
             N.call('tct_pairs_integrate', self.kid, q[0].shape[0], N.ptr(q[0]), N.ptr(q[1]),
                    N.ptr(self.gpu_pts), N.ptr(self.gpu_tris), e - s, N.ptr(pairs_dev[s:e]),
-                   width, int(bool(check0)), self.params.ctypes.data, lanes_per_pair,
+                   width, r0_test_mode(q, check0), self.params.ctypes.data, lanes_per_pair,
                    N.ptr(out[s:e]), st)
         return out
 

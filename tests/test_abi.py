@@ -41,7 +41,7 @@ def test_argument_errors_are_reported_without_a_device():
     assert rc < 0
     assert b'kernel' in L.tct_last_error()
     # unknown launch-shape flag bits are refused before anything runs
-    bad = N.TileCfg(256, 1, 1, 4)
+    bad = N.TileCfg(256, 1, 1, 8)
     buf = ctypes.c_double(0.0)
     rc = L.tct_far_p2p(3, 0, 64, ctypes.byref(buf), ctypes.byref(buf), None, 64,
                        ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf), 0.0, 0.0,
@@ -54,6 +54,8 @@ def test_argument_errors_are_reported_without_a_device():
                            ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf), eps2, ext,
                            ctypes.byref(ok), ctypes.byref(buf), None)
         assert rc < 0 and (b'eps2' in L.tct_last_error() or b'extent' in L.tct_last_error())
+    rc = L.tct_calib_valu(9, 1, 1, ctypes.byref(buf), None)
+    assert rc < 0 and b'mode' in L.tct_last_error()
     n = ctypes.c_int64(-1)
     rc = L.tct_find_near_pairs(0, None, None, 0, None, None, 1.0, None, 0, ctypes.byref(n))
     assert rc == 0 and n.value == 0

@@ -146,7 +146,8 @@ def test_cfg1_u_farfield(oracle, prec):
 
 
 @pytest.mark.parametrize('cfg', [(64, 1, 1, 0), (128, 2, 3, 0), (256, 4, 7, 0), (256, 1, 32, 0),
-                                 (256, 1, 8, 2), (128, 2, 16, 2), (64, 4, 24, 2)])
+                                 (256, 1, 8, 2), (128, 2, 16, 2), (64, 4, 24, 2), (64, 1, 24, 6),
+                                 (256, 2, 5, 4), (128, 4, 3, 5)])
 def test_tile_shapes_agree(cfg):
     # every launch shape computes the same sum; FP64 results agree to
     # roundoff; the XCD chunk order (flags 2) gives the bits of the plain
@@ -242,7 +243,8 @@ def test_tiles_without_the_r2_test_change_nothing(K):
     # source tiles far from a wave's observation groups skip the r2 <= eps2
     # test; forcing the test everywhere (flags bit 0) must give the same
     # bits, for one point set packed twice and for overlapping subsets
-    # packed differently (shared points in unrelated groups)
+    # packed differently (shared points in unrelated groups); so must the
+    # records staged through LDS (bit 2), with and without the test
     from tectosaur_amd import _native as N
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
     pts, tris = cfg1_mesh()
@@ -250,13 +252,14 @@ def test_tiles_without_the_r2_test_change_nothing(K):
     for obs, src in ((np.arange(n), np.arange(n)), (np.arange(n // 2 + 300),
                                                     np.arange(n // 2 - 300, n))):
         outs = []
-        for flags in (0, 1):
+        for flags in (0, 1, 4, 5):
             op = TriToTriDirectFarfieldOp(2, K, [3e10, 0.25], pts, tris, np.float32, obs, src,
                                           tile_cfg=N.TileCfg(128, 1, 4, flags))
             v = np.random.default_rng(4).normal(size=op.shape[1])
             outs.append(op.dot(v))
         assert np.all(np.isfinite(outs[0]))
-        assert np.array_equal(outs[0], outs[1]), (K, obs.shape[0], src.shape[0])
+        for o in outs[1:]:
+            assert np.array_equal(outs[0], o), (K, obs.shape[0], src.shape[0])
 
 
 @pytest.mark.parametrize('K', ['elasticH3', 'elasticRH3'])

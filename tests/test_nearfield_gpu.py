@@ -85,6 +85,34 @@ def test_lanes_per_pair_agree(oracle, lanes, K):
     assert ok, err
 
 
+@pytest.mark.parametrize('K', ['elasticH3', 'elasticRH3'])
+def test_r0_test_modes_agree_on_rules_without_equal_points(K):
+    # the singular rules have no point of equal obs and src parameters, so
+    # testing r2 == 0 alone (mode 2, what the integrator asks for with these
+    # rules) gives the bits of the full test (mode 1), for every lane count
+    import torch
+    from tectosaur_amd import _native as N
+    from tectosaur_amd.pairs_integrator import PairsIntegrator
+    pts, tris = small_mesh(True)
+    co, ea, va, near = classes(pts, tris)
+    pi = PairsIntegrator(K, PARAMS, np.float64, 2, 3, pts, tris)
+    for prs, q in ((co, triangle_rules.coincident_quad(3)), (ea, triangle_rules.edge_adj_quad(3)),
+                   (va, triangle_rules.vertex_adj_quad(3, 3, 3))):
+        qd = pi.quad_to_gpu(q)
+        assert not qd.has_equal_params
+        p = torch.as_tensor(prs.astype(np.int32), device='cuda')
+        for lanes in (1, 4, 16, 64):
+            outs = []
+            for mode in (1, 2):
+                o = torch.empty((prs.shape[0], 81), dtype=torch.float64, device='cuda')
+                N.call('tct_pairs_integrate', pi.kid, qd[0].shape[0], N.ptr(qd[0]),
+                       N.ptr(qd[1]), N.ptr(pi.gpu_pts), N.ptr(pi.gpu_tris), prs.shape[0],
+                       N.ptr(p), p.shape[1], mode, pi.params.ctypes.data, lanes, N.ptr(o),
+                       N.stream_ptr())
+                outs.append(o)
+            assert torch.equal(outs[0], outs[1]), (K, lanes)
+
+
 def test_golden_two_tris_on_device():
     from tectosaur_amd.pairs_integrator import PairsIntegrator
     pts = np.array([[1, 0, 0], [2, 0, 0], [1, 1, 0], [5, 0, 0], [6, 0, 0], [5, 1, 0]], float)

@@ -39,3 +39,26 @@ def test_adjacent_rules_stay_away_from_r0(n):
     for (q, _), S in ((T.edge_adj_quad(n), ea), (T.vertex_adj_quad(n, n, n), va)):
         d = np.linalg.norm(_phys(TRI, q[:, 0], q[:, 1]) - _phys(S, q[:, 2], q[:, 3]), axis=1)
         assert d.min() > 1e-5
+
+
+def test_r0_test_mode_follows_the_rule():
+    # the product rule has points with equal obs and src parameters (on a
+    # coincident pair those are at r = 0: the full test); the singular rules
+    # have none, so r2 == 0 alone is tested
+    from tectosaur_amd.pairs_integrator import r0_test_mode, rule_has_equal_params
+    from tectosaur_amd.quadrature import gauss4d_tri
+
+    class Rule(tuple):
+        pass
+
+    for q, eq in ((gauss4d_tri(2, 2), True), (T.coincident_quad(3), False),
+                  (T.coincident_quad(8), False),
+                  (T.edge_adj_quad(8), False),
+                  (T.vertex_adj_quad(8, 8, 8), False)):
+        assert rule_has_equal_params(q[0]) == eq
+        r = Rule((None, None))
+        r.has_equal_params = eq
+        assert r0_test_mode(r, True) == (1 if eq else 2)
+        assert r0_test_mode(r, False) == 0
+    # a rule without the attribute gets the full test
+    assert r0_test_mode((None, None), True) == 1
