@@ -514,16 +514,22 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
     from tectosaur_amd.traction_slip import random_requests
     warm = random_requests(2, rate=0.0, seed=seed + 99, n_min=512, n_max=1024)
     for m in modes:
-        Server(mode=m, max_active=2).run(warm, realtime=False)
+        base_mode, _, opt = m.partition('+')
+        Server(mode=base_mode, max_active=2, batch_nbody=(opt == 'nbody')).run(warm,
+                                                                              realtime=False)
     Server(mode='f64', max_active=1, prep_workers=0, graphs=False).run(warm, realtime=False)
     reqs = random_requests(n_req, rate=rate, seed=seed, n_min=min_tris, n_max=max_tris)
     out = {'workload': '%d fault requests, Poisson %.2f/s, strike U(0,360) dip U(20,90) '
                        'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near and'
                        ' far, tol 1e-8' % (n_req, rate, min_tris, max_tris)}
-    runs = [('batched_' + m, m, max_active, 4) for m in ('tree', 'mixed', 'f64') if m in modes]
+    # a mode with the suffix '+nbody' runs the FP32 direct-farfield N-bodies
+    # of each tick's solves as one segment-table launch (Server batch_nbody)
+    runs = [('batched_' + m.replace('+', '_'), m, max_active, 4) for m in modes if m != 'seq']
     runs.append(('sequential_f64', 'f64', 1, 0))
     for name, mode, ma, pw in runs:
-        srv = Server(mode=mode, max_active=ma, prep_workers=pw, graphs=(ma > 1))
+        base_mode, _, opt = mode.partition('+')
+        srv = Server(mode=base_mode, max_active=ma, prep_workers=pw, graphs=(ma > 1),
+                     batch_nbody=(opt == 'nbody'))
         torch.cuda.synchronize()
         recs = srv.run(reqs, realtime=True)
         sm = summarize(recs, srv.makespan)
@@ -552,7 +558,8 @@ def main():
     ap.add_argument('--serve-requests', type=int, default=256)
     ap.add_argument('--serve-modes', default='tree,mixed,f64',
                     help="batched modes to run beside the sequential FP64 path, from "
-                         "'tree', 'mixed', 'f64'")
+                         "'tree', 'mixed', 'f64'; '+nbody' (e.g. 'mixed+nbody') batches "
+                         "each tick's FP32 direct-farfield N-bodies into one launch")
     ap.add_argument('--serve-max-tris', type=int, default=32768)
     ap.add_argument('--serve-rate', type=float, default=100.0)
     ap.add_argument('--no-serve', action='store_true')

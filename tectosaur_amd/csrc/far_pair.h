@@ -22,14 +22,14 @@ struct KConst {
 
 // Without the test, for tiles where no pair can reach r2 <= eps2.
 __device__ __forceinline__ float rsq_plain(float r2) { return __builtin_amdgcn_rsqf(r2); }
-__device__ __forceinline__ double rsq_plain(double r2) { return rsqrt(r2); }
+__device__ __forceinline__ double rsq_plain(double r2) { return rsq_pos(r2); }
 
 __device__ __forceinline__ float rsq_masked(float r2, float eps2) {
     float v = __builtin_amdgcn_rsqf(r2);
     return r2 > eps2 ? v : 0.0f;
 }
 __device__ __forceinline__ double rsq_masked(double r2, double eps2) {
-    double v = rsqrt(r2);
+    double v = rsq_pos(r2);
     return r2 > eps2 ? v : 0.0;
 }
 

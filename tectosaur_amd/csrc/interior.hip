@@ -89,18 +89,22 @@ __device__ __forceinline__ double final_pt(const double *acc, const Tri &st, int
 // Writes the 27 outputs of one pair: out[d1*9 + b2*3 + d2] =
 // J res[d1*9 + derot(b2)*3 + d2] with derot(b) = (b - clicks) mod 3
 // (assemble.cl:157-175); with finite_part only b2 == clicks is nonzero.
+// Written as the inverse map on the output address, res basis b going to
+// b2 = (b + clicks) mod 3, so the accumulators are read at static indices
+// (read at the rotated index they would be held in scratch memory); with
+// finite_part only b = 0 is kept.
 template <typename F>
 __device__ __forceinline__ void write_derotated(double *o, int clicks, int finite_part, double J,
                                                 F value) {
 #pragma unroll
-    for (int b2 = 0; b2 < 3; ++b2) {
-        const int src_b = pmod3(b2 - clicks);
-        const bool keep = !finite_part || b2 == clicks;
+    for (int b = 0; b < 3; ++b) {
+        const int b2 = pmod3(b + clicks);
+        const bool keep = !finite_part || b == 0;
 #pragma unroll
         for (int d1 = 0; d1 < 3; ++d1)
 #pragma unroll
             for (int d2 = 0; d2 < 3; ++d2)
-                o[d1 * 9 + b2 * 3 + d2] = keep ? J * value(d1, src_b, d2) : 0.0;
+                o[d1 * 9 + b2 * 3 + d2] = keep ? J * value(d1, b, d2) : 0.0;
     }
 }
 

@@ -46,16 +46,6 @@ __device__ __forceinline__ void pair_geom(const Tri &ot, const Tri &st, PairGeom
     }
 }
 
-// 1/sqrt(x) for finite x > 0: the hardware estimate refined by one
-// third-order Newton step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2, which
-// takes the estimate's error to below double rounding.  No special-value
-// handling: x = 0 gives a non-finite value, which the r = 0 test replaces.
-__device__ __forceinline__ double rsq_pos(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    const double e = fma(-(x * y), y, 1.0);
-    return fma(y * e, fma(e, 0.375, 0.5), y);
-}
-
 // The r = 0 test of a pair class (CK): CK_NONE for pairs that never touch;
 // CK_FULL skips r2 == 0 and the points whose obs and src parameters are
 // equal on pairs that are one triangle in one frame; CK_R2 skips r2 == 0

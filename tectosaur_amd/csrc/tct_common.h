@@ -29,6 +29,17 @@ int set_error(int code, const char *fmt, ...);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// 1/sqrt(x) in FP64 for finite x > 0: the hardware estimate refined by one
+// third-order Newton step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2, which
+// takes the estimate's error below double rounding.  No special-value
+// handling (3 instructions fewer than the library routine): x = 0 gives a
+// non-finite value, which callers replace by 0.
+__device__ __forceinline__ double rsq_pos(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y), y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Kernel material constants (tectosaur/kernels/__init__.py constants_code).

@@ -133,6 +133,12 @@ def _prepare(req, orders, tree=None, omp_threads=None):
     return prep
 
 
+# Solves below this many triangles have their FP32 direct-farfield N-bodies
+# batched per tick when the server's batch_nbody is set (the size below
+# which the tree mode uses the direct farfield).
+NBODY_BATCH_MAX_TRIS = 3000
+
+
 def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None,
            capture_lock=None, batch_nbody=False):
     precisions = ('f64',) if mode == 'f64' else ('f32', 'f64')
@@ -146,9 +152,13 @@ def _build(prep, mode, tol, restart, maxiter, graphs=False, capture_stream=None,
     state = make_state(op, b, mode=mode, tol=tol, restart=restart, maxiter=maxiter)
     if graphs:
         precs = MODES[mode][0:2] + MODES[mode][3:4]
-        # with batched N-bodies, the inner (FP32 direct) applies are split
-        # graphs around the N-body; the other precisions keep whole graphs
-        split = batch_nbody and mode == 'mixed'
+        # with batched N-bodies, the inner (FP32 direct) applies of small
+        # solves are split graphs around the N-body; the other precisions,
+        # and larger solves, keep whole graphs: a large solve's N-body fills
+        # the device with its own tuned launch shape, and its stream
+        # overlaps the others (on the 256-request stream, batching every
+        # mixed-mode solve measured 11% fewer solves/s)
+        split = batch_nbody and mode == 'mixed' and prob.n_tris < NBODY_BATCH_MAX_TRIS
         whole = tuple(p for p in precs if not (split and p == MODES[mode][0]))
         with capture_lock or contextlib.nullcontext():
             op.enable_graphs(whole, stream=capture_stream)

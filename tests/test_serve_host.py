@@ -72,6 +72,7 @@ def test_build_captures_under_the_capture_lock(monkeypatch):
     class Prob:
         def __init__(self, *a, **kw):
             self.H, self.cm, self.rhs = H(), None, None
+            self.n_tris = 500   # below the size up to which N-bodies are batched
 
     class CmT:
         def dot_dev(self, x):
@@ -107,6 +108,20 @@ def test_build_captures_under_the_capture_lock(monkeypatch):
     assert seen['split_held'] is True and seen['split_precs'] == ('f32',)
     assert 'f32' not in seen['precs'] and st.batch_split
     assert not lock.held()
+    # a larger direct-farfield solve keeps whole graphs (its own N-body launch)
+    seen.clear()
+    big = Prob
+    monkeypatch.setattr(serve, 'TractionSlipProblem',
+                        lambda *a, **kw: _with(big(), n_tris=serve.NBODY_BATCH_MAX_TRIS))
+    _, _, st = serve._build({}, 'tree', 1e-8, 100, 1000, graphs=True, capture_stream=None,
+                            capture_lock=lock, batch_nbody=True)
+    assert not st.batch_split and 'split_precs' not in seen and 'f32' in seen['precs']
+
+
+def _with(obj, **kw):
+    for k, v in kw.items():
+        setattr(obj, k, v)
+    return obj
 
 
 def test_estimate_bytes_grows_with_size_and_tree():
