@@ -1,5 +1,5 @@
 """Operator combinators (tectosaur/ops/sum_op.py, neg_op.py, composite_op.py,
-dense_op.py): the reference's own tests (tests/test_op_combinators.py:6-28)
+dense_op.py): the reference's own tests (tests/test_op_combinators.py:6-27)
 plus the device product ``dot_dev`` on torch tensors (CPU tensors here)."""
 import numpy as np
 import pytest
