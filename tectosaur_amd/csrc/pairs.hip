@@ -207,6 +207,81 @@ __device__ __forceinline__ void fence_point(double last, double (&b)[3], double 
     asm volatile("" : "+v"(b[0]), "+v"(b[1]), "+v"(b[2]), "+v"(dep) : "s"(last) : "memory");
 }
 
+// The 81 outputs of one pair from its reduced accumulators: the triangle
+// records are formed again, from indices the compiler cannot match with the
+// first loads (it would otherwise keep those records live through the rule
+// loop), and the de-rotation of assemble.cl:77-101 is applied: out[b1] =
+// res[od[b1]] with od[i] = (i - clicks) mod 3, entries 0 and 1 of the source
+// map swapped when the source was flipped -- written as the inverse map on
+// the output address, so every register index stays static.
+template <int KID>
+__device__ __forceinline__ void write_pair_output(const double *acc,
+                                                  const double *__restrict__ pts,
+                                                  const int32_t *__restrict__ tris,
+                                                  const int32_t *pp, int oc, int sc, int flip,
+                                                  const ElasticConsts &c, double *o) {
+    int oi = pp[0], si = pp[1];
+    asm volatile("" : "+v"(oi), "+v"(si));
+    Tri ot, st;
+    load_tri(ot, pts, tris, oi, oc, 0);
+    load_tri(st, pts, tris, si, sc, flip);
+    int sd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sd[i] = pmod3(i - sc);
+    if (flip) {
+        const int tmp = sd[0];
+        sd[0] = sd[1];
+        sd[1] = tmp;
+    }
+    int oinv[3], sinv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        oinv[i] = pmod3(i + oc);
+        sinv[i] = (sd[0] == i) ? 0 : ((sd[1] == i) ? 1 : 2);
+    }
+    const double J = ot.jac * st.jac;
+#pragma unroll
+    for (int i1 = 0; i1 < 3; ++i1)
+#pragma unroll
+        for (int d1 = 0; d1 < 3; ++d1)
+#pragma unroll
+            for (int i2 = 0; i2 < 3; ++i2)
+#pragma unroll
+                for (int d2 = 0; d2 < 3; ++d2)
+                    o[oinv[i1] * 27 + d1 * 9 + sinv[i2] * 3 + d2] =
+                        J * final_value<KID>(acc, ot, st, c, i1, d1, i2, d2);
+}
+
+// One lane walks rule points [q0, q1) of its pair; every lane of the wave
+// walks the same points, read through wave-uniform scalar loads.  Those
+// return out of order, so a wait for one waits for all: the next point is
+// loaded while the current one computes, and fence_point puts the one wait
+// per point after the previous point's arithmetic and before the next load
+// is issued and the arithmetic that uses the point.
+template <int KID, int CK>
+__device__ __forceinline__ void rule_loop_uniform(int q0, int q1, const double *__restrict__ qx,
+                                                  const double *__restrict__ qw,
+                                                  const PairNormals &nm, PairGeom &g,
+                                                  const ElasticConsts &c, double *acc,
+                                                  bool same) {
+    constexpr int NA = n_acc<KID>();
+    double qa[5], qb[5];
+    load_point(qa, qx, qw, q0);
+    int iq = q0;
+    for (; iq + 1 < q1; iq += 2) {
+        fence_point(qa[4], g.b, acc[NA - 1]);
+        load_point(qb, qx, qw, iq + 1);
+        accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
+        fence_point(qb[4], g.b, acc[NA - 1]);
+        load_point(qa, qx, qw, iq + 2 < q1 ? iq + 2 : q1 - 1);
+        accumulate_point<KID, CK>(nm, g, qb, qb[4], c, acc, same);
+    }
+    if (iq < q1) {   // the last point, loaded by the final clamp
+        fence_point(qa[4], g.b, acc[NA - 1]);
+        accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
+    }
+}
+
 template <int KID, int GL, int CK>
 __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__restrict__ qx,
                                                     const double *__restrict__ qw,
@@ -252,28 +327,8 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
     if constexpr (GL == 1) {
-        // one pair per lane: every lane walks the same rule, whose points are
-        // wave-uniform scalar loads.  Those return out of order, so a wait
-        // for one waits for all: the next point is loaded while the current
-        // one computes, and fence_point puts the one wait per point after
-        // the previous point's arithmetic and before the next load is issued
-        // and the arithmetic that uses the point.
-        if (live && nq > 0) {
-            double qa[5], qb[5];
-            load_point(qa, qx, qw, 0);
-            for (int iq = 0; iq + 1 < nq; iq += 2) {
-                fence_point(qa[4], g.b, acc[NA - 1]);
-                load_point(qb, qx, qw, iq + 1);
-                accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
-                fence_point(qb[4], g.b, acc[NA - 1]);
-                load_point(qa, qx, qw, iq + 2 < nq ? iq + 2 : nq - 1);
-                accumulate_point<KID, CK>(nm, g, qb, qb[4], c, acc, same);
-            }
-            if (nq & 1) {   // the last point, loaded by the final clamp
-                fence_point(qa[4], g.b, acc[NA - 1]);
-                accumulate_point<KID, CK>(nm, g, qa, qa[4], c, acc, same);
-            }
-        }
+        // one pair per lane: every lane walks the same rule
+        if (live && nq > 0) rule_loop_uniform<KID, CK>(0, nq, qx, qw, nm, g, c, acc, same);
     } else if (live) {
         // lanes of a group take every GL-th point (consecutive lanes read
         // consecutive points: coalesced vector loads).  The first nfull
@@ -311,46 +366,81 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
         }
     }
     if (!live || sub != 0) return;
-    // the triangle records again, from indices the compiler cannot match
-    // with the first loads (it would otherwise keep those records live
-    // through the rule loop)
-    int oi = pp[0], si = pp[1];
-    asm volatile("" : "+v"(oi), "+v"(si));
-    Tri ot, st;
-    load_tri(ot, pts, tris, oi, oc, 0);
-    load_tri(st, pts, tris, si, sc, flip);
+    write_pair_output<KID>(acc, pts, tris, pp, oc, sc, flip, c, out + (size_t)pair * 81);
+}
 
-    // De-rotation (assemble.cl:77-101): out[b1] = res[od[b1]] with
-    // od[i] = (i - clicks) mod 3; for the source, entries 0 and 1 of the
-    // map are swapped when the source was flipped.  Written here as the
-    // inverse map applied to the output address, so every register index
-    // stays static.
-    int sd[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) sd[i] = pmod3(i - sc);
-    if (flip) {
-        const int tmp = sd[0];
-        sd[0] = sd[1];
-        sd[1] = tmp;
+// RH3 pairs, one lane per (pair, rule chunk): grid (pair blocks, chunks);
+// chunk ch takes rule points [nq ch / C, nq (ch + 1) / C), read through
+// wave-uniform scalar loads as with one lane per pair, so pairs too few to
+// fill the device one lane each still do.  The lane's 7 accumulators go to
+// its pair's own output row, out[pair][7 ch .. 7 ch + 7) (C <= 11), and
+// pairs_rh_finish_kernel sums the chunks in order and writes the 81 values.
+constexpr int RH_MAX_CHUNKS = 81 / 7;   // chunk accumulators that fit in an output row
+
+template <int CK>
+__global__ __launch_bounds__(256) void pairs_rh_chunk_kernel(
+    int nq, const double *__restrict__ qx, const double *__restrict__ qw,
+    const double *__restrict__ pts, const int32_t *__restrict__ tris, int n_pairs,
+    const int32_t *__restrict__ pairs, int pair_width, ElasticConsts c, double *__restrict__ out) {
+    constexpr int NA = n_acc<TCT_RH>();
+    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pair >= n_pairs) return;
+    const int ch = (int)blockIdx.y, nch = (int)gridDim.y;
+    const int32_t *pp = pairs + (size_t)pair * pair_width;
+    int oc = 0, sc = 0, flip = 0;
+    if (pair_width == 5) {
+        oc = pp[2];
+        sc = pp[3];
+        flip = pp[4] != 0;
     }
-    int oinv[3], sinv[3];
+    PairGeom g;
+    PairNormals nm;
+    {
+        Tri ot, st;
+        load_tri(ot, pts, tris, pp[0], oc, 0);
+        load_tri(st, pts, tris, pp[1], sc, flip);
+        pair_geom(ot, st, g);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        oinv[i] = pmod3(i + oc);
-        sinv[i] = (sd[0] == i) ? 0 : ((sd[1] == i) ? 1 : 2);
+        for (int d = 0; d < 3; ++d) {
+            nm.m[d] = ot.n[d];
+            nm.n[d] = st.n[d];
+        }
     }
-    const double J = ot.jac * st.jac;
+    const bool same = pp[0] == pp[1] && oc == sc && !flip;
+    double acc[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    const int q0 = (int)((int64_t)nq * ch / nch), q1 = (int)((int64_t)nq * (ch + 1) / nch);
+    if (q1 > q0) rule_loop_uniform<TCT_RH, CK>(q0, q1, qx, qw, nm, g, c, acc, same);
+    double *o = out + (size_t)pair * 81 + NA * ch;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) o[k] = acc[k];
+}
+
+__global__ __launch_bounds__(256) void pairs_rh_finish_kernel(
+    int n_pairs, const double *__restrict__ pts, const int32_t *__restrict__ tris,
+    const int32_t *__restrict__ pairs, int pair_width, int nch, ElasticConsts c,
+    double *__restrict__ out) {
+    constexpr int NA = n_acc<TCT_RH>();
+    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pair >= n_pairs) return;
+    const int32_t *pp = pairs + (size_t)pair * pair_width;
+    int oc = 0, sc = 0, flip = 0;
+    if (pair_width == 5) {
+        oc = pp[2];
+        sc = pp[3];
+        flip = pp[4] != 0;
+    }
     double *o = out + (size_t)pair * 81;
+    // every chunk's accumulators are read (in chunk order) before the row
+    // is overwritten with the outputs
+    double acc[NA];
 #pragma unroll
-    for (int i1 = 0; i1 < 3; ++i1)
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    for (int ch = 0; ch < nch; ++ch)
 #pragma unroll
-        for (int d1 = 0; d1 < 3; ++d1)
-#pragma unroll
-            for (int i2 = 0; i2 < 3; ++i2)
-#pragma unroll
-                for (int d2 = 0; d2 < 3; ++d2)
-                    o[oinv[i1] * 27 + d1 * 9 + sinv[i2] * 3 + d2] =
-                        J * final_value<KID>(acc, ot, st, c, i1, d1, i2, d2);
+        for (int k = 0; k < NA; ++k) acc[k] += o[NA * ch + k];
+    write_pair_output<TCT_RH>(acc, pts, tris, pp, oc, sc, flip, c, o);
 }
 
 __global__ void sub_f64_kernel(int64_t n, const double *__restrict__ a,
@@ -410,7 +500,13 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
                   "pairs: null pointer");
     if (n_pairs == 0) return TCT_OK;
     int gl = lanes_per_pair;
-    if (gl == 0) {
+    int chunks = 0;   // RH3 rule chunks (one lane per pair and chunk)
+    if (gl < 0) {
+        TCT_CHECK_ARG(kernel == TCT_RH && -gl <= RH_MAX_CHUNKS && -gl <= nq,
+                      "pairs: %d rule chunks (RH3 only, at most %d and nq)", -gl, RH_MAX_CHUNKS);
+        chunks = -gl;
+        gl = 1;
+    } else if (gl == 0) {
         // Fewest lanes per pair that still give enough wavefronts, at most
         // what the rule size can use.  One lane per pair reads the rule
         // through wave-uniform scalar loads: on 1.7M near pairs (4,096-point
@@ -426,12 +522,36 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
                 gl = g;
                 break;
             }
+        // RH3 with a large rule: one lane per pair and rule chunk instead of
+        // several lanes per pair, enough chunks (at most RH_MAX_CHUNKS) for
+        // the wavefronts the lane choice above asks for
+        if (kernel == TCT_RH && gl > 1 && nq >= 512) {
+            const int64_t want = ceil_div(64LL * min_waves, n_pairs);
+            chunks = (int)(want < RH_MAX_CHUNKS ? (want < 2 ? 2 : want) : RH_MAX_CHUNKS);
+            gl = 1;
+        }
     }
     TCT_CHECK_ARG(gl == 1 || gl == 4 || gl == 16 || gl == 64, "pairs: lanes_per_pair %d", gl);
     TCT_CHECK_ARG(n_pairs * gl < (1LL << 31), "pairs: n_pairs * lanes_per_pair >= 2^31");
     const ElasticConsts c = elastic_consts(params_host);
     hipStream_t st = as_stream(stream);
     const int np = (int)n_pairs, q = (int)nq;
+    if (chunks > 0) {
+        const dim3 grid((unsigned)ceil_div(n_pairs, (int64_t)256), (unsigned)chunks);
+        if (check0 == CK_FULL)
+            hipLaunchKernelGGL(pairs_rh_chunk_kernel<CK_FULL>, grid, dim3(256), 0, st, q, qx, qw,
+                               pts, tris, np, pairs, pair_width, c, out);
+        else if (check0 == CK_R2)
+            hipLaunchKernelGGL(pairs_rh_chunk_kernel<CK_R2>, grid, dim3(256), 0, st, q, qx, qw,
+                               pts, tris, np, pairs, pair_width, c, out);
+        else
+            hipLaunchKernelGGL(pairs_rh_chunk_kernel<CK_NONE>, grid, dim3(256), 0, st, q, qx, qw,
+                               pts, tris, np, pairs, pair_width, c, out);
+        hipLaunchKernelGGL(pairs_rh_finish_kernel, dim3((unsigned)ceil_div(n_pairs, (int64_t)256)),
+                           dim3(256), 0, st, np, pts, tris, pairs, pair_width, chunks, c, out);
+        TCT_CHECK_LAUNCH("pairs_integrate");
+        return TCT_OK;
+    }
     switch (kernel) {
     case TCT_U: launch_pairs_g<TCT_U>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;
     case TCT_T: launch_pairs_g<TCT_T>(gl, st, q, qx, qw, pts, tris, np, pairs, pair_width, check0, c, out); break;

@@ -85,6 +85,28 @@ def test_lanes_per_pair_agree(oracle, lanes, K):
     assert ok, err
 
 
+@pytest.mark.parametrize('chunks', [1, 2, 5, 11])
+def test_rh_rule_chunks_vs_oracle(oracle, chunks):
+    # RH3 one lane per pair and rule chunk (lanes_per_pair = -chunks): the
+    # chunks' sums added in order reproduce the integrals, for the singular
+    # classes (flipped sources included) and a rule that does not split
+    # evenly
+    import torch
+    from tectosaur_amd.pairs_integrator import PairsIntegrator
+    pts, tris = small_mesh(True)
+    co, ea, va, near = classes(pts, tris)
+    pi = PairsIntegrator('elasticRH3', PARAMS, np.float64, 2, 3, pts, tris)
+    for prs, q in ((co[:17], triangle_rules.coincident_quad(3)),
+                   (ea[:37], triangle_rules.edge_adj_quad(3)),
+                   (va[:29], triangle_rules.vertex_adj_quad(3, 3, 3))):
+        qd = pi.quad_to_gpu(q)
+        p = torch.as_tensor(prs.astype(np.int32), device='cuda')
+        got = pi.pairs_quad_dev(qd, p, True, lanes_per_pair=-chunks).cpu().numpy()
+        want = oracle.pairs_integrate('elasticRH3', q[0], q[1], pts, tris, prs, True, PARAMS)
+        ok, err = blockwise_close(got, want)
+        assert ok, (chunks, prs.shape, err)
+
+
 @pytest.mark.parametrize('K', ['elasticH3', 'elasticRH3'])
 def test_r0_test_modes_agree_on_rules_without_equal_points(K):
     # the singular rules have no point of equal obs and src parameters, so
