@@ -500,7 +500,7 @@ def cpu_baseline(pts, tris, v, rows, out64_rows, out32_rows, seconds=8.0):
 
 
 def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
-                modes=('tree', 'mixed', 'f64')):
+                modes=('tree', 'mixed', 'f64'), baseline=True):
     """Served traction -> slip solves: one Poisson stream of fault requests
     through the continuous-batching server and through the plain FP64
     one-at-a-time path (always run, the baseline); latency = finish -
@@ -517,7 +517,8 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
         base_mode, _, opt = m.partition('+')
         Server(mode=base_mode, max_active=2, batch_nbody=(opt == 'nbody')).run(warm,
                                                                               realtime=False)
-    Server(mode='f64', max_active=1, prep_workers=0, graphs=False).run(warm, realtime=False)
+    if baseline:
+        Server(mode='f64', max_active=1, prep_workers=0, graphs=False).run(warm, realtime=False)
     reqs = random_requests(n_req, rate=rate, seed=seed, n_min=min_tris, n_max=max_tris)
     out = {'workload': '%d fault requests, Poisson %.2f/s, strike U(0,360) dip U(20,90) '
                        'length U(5,50) km aspect U(1,4), %d-%d tris log-uniform, RH3 near and'
@@ -525,7 +526,8 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
     # a mode with the suffix '+nbody' runs the FP32 direct-farfield N-bodies
     # of each tick's solves as one segment-table launch (Server batch_nbody)
     runs = [('batched_' + m.replace('+', '_'), m, max_active, 4) for m in modes if m != 'seq']
-    runs.append(('sequential_f64', 'f64', 1, 0))
+    if baseline:
+        runs.append(('sequential_f64', 'f64', 1, 0))
     for name, mode, ma, pw in runs:
         base_mode, _, opt = mode.partition('+')
         srv = Server(mode=base_mode, max_active=ma, prep_workers=pw, graphs=(ma > 1),
@@ -534,8 +536,11 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
         recs = srv.run(reqs, realtime=True)
         sm = summarize(recs, srv.makespan)
         sm['ticks'] = srv.ticks
+        sm['loop_wait_s'] = srv.sync_wait_s   # ticking thread waiting for the device
         out[name] = sm
         print('# serve %s: %s' % (name, json.dumps(sm)), flush=True)
+    if not baseline:
+        return out
     f = out['sequential_f64']
     for name, _, _, _ in runs[:-1]:
         b = out[name]
@@ -563,6 +568,8 @@ def main():
     ap.add_argument('--serve-max-tris', type=int, default=32768)
     ap.add_argument('--serve-rate', type=float, default=100.0)
     ap.add_argument('--no-serve', action='store_true')
+    ap.add_argument('--no-serve-baseline', action='store_true',
+                    help='skip the sequential FP64 served run (for traces of the batched path)')
     ap.add_argument('--no-nearfield', action='store_true')
     ap.add_argument('--no-treecode', action='store_true')
     ap.add_argument('--no-interior', action='store_true')
@@ -677,7 +684,8 @@ def main():
         line['krylov'] = krylov_bench()
     if not args.no_serve:
         line['served'] = serve_bench(args.serve_requests, args.serve_max_tris, args.serve_rate,
-                                     modes=tuple(m for m in args.serve_modes.split(',') if m))
+                                     modes=tuple(m for m in args.serve_modes.split(',') if m),
+                                     baseline=not args.no_serve_baseline)
     if not args.no_cpu_baseline:
         rng = np.random.default_rng(1)
         rows = np.sort(rng.choice(n, size=min(args.cpu_rows, n), replace=False))
