@@ -369,24 +369,35 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
     write_pair_output<KID>(acc, pts, tris, pp, oc, sc, flip, c, out + (size_t)pair * 81);
 }
 
-// RH3 pairs, one lane per (pair, rule chunk): grid (pair blocks, chunks);
-// chunk ch takes rule points [nq ch / C, nq (ch + 1) / C), read through
-// wave-uniform scalar loads as with one lane per pair, so pairs too few to
-// fill the device one lane each still do.  The lane's 7 accumulators go to
-// its pair's own output row, out[pair][7 ch .. 7 ch + 7) (C <= 11), and
-// pairs_rh_finish_kernel sums the chunks in order and writes the 81 values.
+// RH3 pairs, one lane per (pair, rule chunk): grid (pair blocks, C slots).
+// With S = 1 a workgroup holds 256 pairs and slot ch is rule chunk ch of C,
+// points [nq ch / C, nq (ch + 1) / C).  With S = 4 a workgroup holds 64
+// pairs, its wave w takes chunk S ch + w of S C, and the waves' sums are
+// added in wave order through LDS, so a slot holds four chunks.  The rule is
+// read through wave-uniform scalar loads as with one lane per pair, so pairs
+// too few to fill the device one lane each still do.  The 7 accumulators of
+// slot ch go to the pair's own output row, out[pair][7 ch .. 7 ch + 7)
+// (C <= 11), and pairs_rh_finish_kernel sums the slots in order and writes
+// the 81 values.
 constexpr int RH_MAX_CHUNKS = 81 / 7;   // chunk accumulators that fit in an output row
+constexpr int RH_WAVE_SPLIT = 4;        // chunks per slot when the waves split one
 
-template <int CK>
+template <int CK, int S>
 __global__ __launch_bounds__(256) void pairs_rh_chunk_kernel(
     int nq, const double *__restrict__ qx, const double *__restrict__ qw,
     const double *__restrict__ pts, const int32_t *__restrict__ tris, int n_pairs,
     const int32_t *__restrict__ pairs, int pair_width, ElasticConsts c, double *__restrict__ out) {
     constexpr int NA = n_acc<TCT_RH>();
-    const int pair = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pair >= n_pairs) return;
+    constexpr int PB = 256 / S;   // pairs per workgroup, a multiple of 64
+    static_assert(PB % 64 == 0, "a wave must not straddle two chunks");
+    const int lane = (int)threadIdx.x % PB;
+    const int w = S > 1 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / PB) : 0;
+    const int pair = (int)blockIdx.x * PB + lane;
+    // lanes past the end work on the last pair (the waves of a workgroup
+    // all reach the LDS barrier) and write nothing
+    const bool live = pair < n_pairs;
+    const int32_t *pp = pairs + (size_t)(live ? pair : n_pairs - 1) * pair_width;
     const int ch = (int)blockIdx.y, nch = (int)gridDim.y;
-    const int32_t *pp = pairs + (size_t)pair * pair_width;
     int oc = 0, sc = 0, flip = 0;
     if (pair_width == 5) {
         oc = pp[2];
@@ -410,8 +421,22 @@ __global__ __launch_bounds__(256) void pairs_rh_chunk_kernel(
     double acc[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0.0;
-    const int q0 = (int)((int64_t)nq * ch / nch), q1 = (int)((int64_t)nq * (ch + 1) / nch);
+    const int64_t cs = (int64_t)ch * S + w, ncs = (int64_t)nch * S;
+    const int q0 = (int)(nq * cs / ncs), q1 = (int)(nq * (cs + 1) / ncs);
     if (q1 > q0) rule_loop_uniform<TCT_RH, CK>(q0, q1, qx, qw, nm, g, c, acc, same);
+    if constexpr (S > 1) {
+        __shared__ double part[S - 1][NA][PB];
+        if (w > 0)
+#pragma unroll
+            for (int k = 0; k < NA; ++k) part[w - 1][k][lane] = acc[k];
+        __syncthreads();
+        if (w > 0) return;
+#pragma unroll
+        for (int s = 0; s < S - 1; ++s)
+#pragma unroll
+            for (int k = 0; k < NA; ++k) acc[k] += part[s][k][lane];
+    }
+    if (!live) return;
     double *o = out + (size_t)pair * 81 + NA * ch;
 #pragma unroll
     for (int k = 0; k < NA; ++k) o[k] = acc[k];
@@ -502,8 +527,12 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
     int gl = lanes_per_pair;
     int chunks = 0;   // RH3 rule chunks (one lane per pair and chunk)
     if (gl < 0) {
-        TCT_CHECK_ARG(kernel == TCT_RH && -gl <= RH_MAX_CHUNKS && -gl <= nq,
-                      "pairs: %d rule chunks (RH3 only, at most %d and nq)", -gl, RH_MAX_CHUNKS);
+        const int64_t cmax = (int64_t)RH_WAVE_SPLIT * RH_MAX_CHUNKS;
+        TCT_CHECK_ARG(kernel == TCT_RH && -(int64_t)gl <= nq &&
+                          (-gl <= RH_MAX_CHUNKS || (-gl <= cmax && -gl % RH_WAVE_SPLIT == 0)),
+                      "pairs: %d rule chunks (RH3 only, at most nq, and at most %d or a "
+                      "multiple of %d up to %d)",
+                      -gl, RH_MAX_CHUNKS, RH_WAVE_SPLIT, (int)cmax);
         chunks = -gl;
         gl = 1;
     } else if (gl == 0) {
@@ -523,11 +552,17 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
                 break;
             }
         // RH3 with a large rule: one lane per pair and rule chunk instead of
-        // several lanes per pair, enough chunks (at most RH_MAX_CHUNKS) for
-        // the wavefronts the lane choice above asks for
+        // several lanes per pair, enough chunks for the wavefronts the lane
+        // choice above asks for: up to RH_MAX_CHUNKS one per slot, beyond
+        // that RH_WAVE_SPLIT per slot (at most RH_MAX_CHUNKS slots)
         if (kernel == TCT_RH && gl > 1 && nq >= 512) {
             const int64_t want = ceil_div(64LL * min_waves, n_pairs);
-            chunks = (int)(want < RH_MAX_CHUNKS ? (want < 2 ? 2 : want) : RH_MAX_CHUNKS);
+            if (want <= RH_MAX_CHUNKS) {
+                chunks = (int)(want < 2 ? 2 : want);
+            } else {
+                const int64_t slots = ceil_div(want, (int64_t)RH_WAVE_SPLIT);
+                chunks = RH_WAVE_SPLIT * (int)(slots < RH_MAX_CHUNKS ? slots : RH_MAX_CHUNKS);
+            }
             gl = 1;
         }
     }
@@ -537,18 +572,26 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
     hipStream_t st = as_stream(stream);
     const int np = (int)n_pairs, q = (int)nq;
     if (chunks > 0) {
-        const dim3 grid((unsigned)ceil_div(n_pairs, (int64_t)256), (unsigned)chunks);
+        const bool split = chunks > RH_MAX_CHUNKS;
+        const int slots = split ? chunks / RH_WAVE_SPLIT : chunks;
+        const dim3 grid((unsigned)ceil_div(n_pairs, (int64_t)(split ? 256 / RH_WAVE_SPLIT : 256)),
+                        (unsigned)slots);
+#define TCT_RH_CHUNKS(CK)                                                                     \
+    if (split)                                                                                \
+        hipLaunchKernelGGL((pairs_rh_chunk_kernel<CK, RH_WAVE_SPLIT>), grid, dim3(256), 0, st, \
+                           q, qx, qw, pts, tris, np, pairs, pair_width, c, out);              \
+    else                                                                                      \
+        hipLaunchKernelGGL((pairs_rh_chunk_kernel<CK, 1>), grid, dim3(256), 0, st, q, qx, qw, \
+                           pts, tris, np, pairs, pair_width, c, out)
         if (check0 == CK_FULL)
-            hipLaunchKernelGGL(pairs_rh_chunk_kernel<CK_FULL>, grid, dim3(256), 0, st, q, qx, qw,
-                               pts, tris, np, pairs, pair_width, c, out);
+            TCT_RH_CHUNKS(CK_FULL);
         else if (check0 == CK_R2)
-            hipLaunchKernelGGL(pairs_rh_chunk_kernel<CK_R2>, grid, dim3(256), 0, st, q, qx, qw,
-                               pts, tris, np, pairs, pair_width, c, out);
+            TCT_RH_CHUNKS(CK_R2);
         else
-            hipLaunchKernelGGL(pairs_rh_chunk_kernel<CK_NONE>, grid, dim3(256), 0, st, q, qx, qw,
-                               pts, tris, np, pairs, pair_width, c, out);
+            TCT_RH_CHUNKS(CK_NONE);
+#undef TCT_RH_CHUNKS
         hipLaunchKernelGGL(pairs_rh_finish_kernel, dim3((unsigned)ceil_div(n_pairs, (int64_t)256)),
-                           dim3(256), 0, st, np, pts, tris, pairs, pair_width, chunks, c, out);
+                           dim3(256), 0, st, np, pts, tris, pairs, pair_width, slots, c, out);
         TCT_CHECK_LAUNCH("pairs_integrate");
         return TCT_OK;
     }
