@@ -25,23 +25,69 @@ constexpr int CB = 8;  // columns per pass of the dot products
 // each wavefront, then the wavefronts in order): column i0 + k of block b
 // goes to partial[(i0 + k) * nb + b] for i0 + k < cols.  Partial sums are
 // stored column by column, [cols][nb], so a column's nb partials are
-// contiguous for the column reduction.
-__device__ __forceinline__ void store_block_sums(double (&s)[CB], double (*red)[CB], int i0,
-                                                 int cols, int nb, double *__restrict__ partial) {
+// contiguous for the column reduction.  red is double-buffered by pass
+// parity: a pass writes red[(i0 / CB) & 1] before its barrier and reads it
+// after, and the next writer of that half comes after the next pass's
+// barrier, so one barrier per pass suffices.
+__device__ __forceinline__ void store_block_sums(double (&s)[CB], double (*red)[KB / 64][CB],
+                                                 int i0, int cols, int nb,
+                                                 double *__restrict__ partial) {
+    double(*r2)[CB] = red[(i0 / CB) & 1];
 #pragma unroll
     for (int k = 0; k < CB; ++k)
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off, 64);
     if (threadIdx.x % 64 == 0)
 #pragma unroll
-        for (int k = 0; k < CB; ++k) red[threadIdx.x / 64][k] = s[k];
+        for (int k = 0; k < CB; ++k) r2[threadIdx.x / 64][k] = s[k];
     __syncthreads();
     if (threadIdx.x < CB && i0 + (int)threadIdx.x < cols) {
         double r = 0.0;
-        for (int wv = 0; wv < KB / 64; ++wv) r += red[wv][threadIdx.x];
+        for (int wv = 0; wv < KB / 64; ++wv) r += r2[wv][threadIdx.x];
         partial[(size_t)(i0 + threadIdx.x) * nb + blockIdx.x] = r;
     }
-    __syncthreads();
+}
+
+// A block's points [t0, t1) are taken in rounds of PU * KB: in a round,
+// thread x holds points tb + x + k KB, k < PU, so every load of a round is
+// issued before the first is used.  A thread's points come in the order
+// t0 + x, t0 + x + KB, ... whatever PU is, so the per-thread sums (and the
+// results) do not depend on PU.
+constexpr int PU = 2;
+
+// The CB dot products of one column pass over the block's points: s[k] +=
+// X_k[t] w[t] for column i0 + k < cols, where X_k = V[i0 + k] for i0 + k <
+// m and X_k = w for i0 + k == m (the norm column).
+__device__ __forceinline__ void dot_pass(double (&s)[CB], int i0, int cols, int m, int64_t ld,
+                                         int64_t t0, int64_t t1,
+                                         const double *__restrict__ V,
+                                         const double *__restrict__ w) {
+#pragma unroll
+    for (int k = 0; k < CB; ++k) s[k] = 0.0;
+    for (int64_t tb = t0; tb < t1; tb += PU * KB) {
+        double wt[PU];
+        bool ok[PU];
+#pragma unroll
+        for (int u = 0; u < PU; ++u) {
+            const int64_t t = tb + u * KB + threadIdx.x;
+            ok[u] = t < t1;
+            wt[u] = ok[u] ? w[t] : 0.0;
+        }
+        double x[PU][CB];
+#pragma unroll
+        for (int u = 0; u < PU; ++u)
+#pragma unroll
+            for (int k = 0; k < CB; ++k) {
+                const int i = i0 + k;
+                const int64_t t = tb + u * KB + threadIdx.x;
+                x[u][k] = (ok[u] && i < m) ? V[(size_t)i * ld + t] : wt[u];
+            }
+#pragma unroll
+        for (int u = 0; u < PU; ++u)
+#pragma unroll
+            for (int k = 0; k < CB; ++k)
+                if (ok[u] && i0 + k < cols) s[k] = fma(x[u][k], wt[u], s[k]);
+    }
 }
 
 // partial[i][b] = sum over block b's points of V[i][t] w[t], i < m, and
@@ -52,7 +98,7 @@ __global__ __launch_bounds__(KB) void multi_dot_kernel(int m, int64_t n, int64_t
                                                        const double *__restrict__ w,
                                                        int with_norm,
                                                        double *__restrict__ partial) {
-    __shared__ double red[KB / 64][CB];
+    __shared__ double red[2][KB / 64][CB];
     const int nb = gridDim.x;
     const int cols = m + (with_norm ? 1 : 0);
     // this block's contiguous range of points
@@ -61,16 +107,7 @@ __global__ __launch_bounds__(KB) void multi_dot_kernel(int m, int64_t n, int64_t
     const int64_t t1 = t0 + per < n ? t0 + per : n;
     for (int i0 = 0; i0 < cols; i0 += CB) {
         double s[CB];
-#pragma unroll
-        for (int k = 0; k < CB; ++k) s[k] = 0.0;
-        for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) {
-            const double wt = w[t];
-#pragma unroll
-            for (int k = 0; k < CB; ++k) {
-                const int i = i0 + k;
-                if (i < cols) s[k] += (i < m ? V[(size_t)i * ld + t] : wt) * wt;
-            }
-        }
+        dot_pass(s, i0, cols, m, ld, t0, t1, V, w);
         store_block_sums(s, red, i0, cols, nb, partial);
     }
 }
@@ -127,7 +164,7 @@ __global__ __launch_bounds__(KB) void reduce_update_dot_kernel(
     const double *__restrict__ add, double *__restrict__ hout, double *__restrict__ w,
     int norm_only, double *__restrict__ partial_out) {
     extern __shared__ double c[];
-    __shared__ double red[KB / 64][CB];
+    __shared__ double red[2][KB / 64][CB];
     const int nb = gridDim.x;
     for (int i = threadIdx.x; i < m; i += KB) {
         if (c_in) {
@@ -143,24 +180,46 @@ __global__ __launch_bounds__(KB) void reduce_update_dot_kernel(
     const int64_t per = (n + nb - 1) / nb;
     const int64_t t0 = (int64_t)blockIdx.x * per;
     const int64_t t1 = t0 + per < n ? t0 + per : n;
-    for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) {
-        double s = 0.0;
-        for (int i = 0; i < m; ++i) s += V[(size_t)i * ld + t] * c[i];
-        w[t] = w[t] - s;
+    // w[t] -= sum_i V[i][t] c[i], i ascending; IU rows' loads of a round's
+    // PU points in flight at once
+    constexpr int IU = 4;
+    for (int64_t tb = t0; tb < t1; tb += PU * KB) {
+        double acc[PU];
+        bool ok[PU];
+#pragma unroll
+        for (int u = 0; u < PU; ++u) {
+            ok[u] = tb + u * KB + threadIdx.x < t1;
+            acc[u] = 0.0;
+        }
+        const double *vp = V + tb + threadIdx.x;
+        int i = 0;
+        for (; i + IU <= m; i += IU) {
+            double x[IU][PU];
+#pragma unroll
+            for (int r = 0; r < IU; ++r)
+#pragma unroll
+                for (int u = 0; u < PU; ++u)
+                    x[r][u] = ok[u] ? vp[(size_t)(i + r) * ld + u * KB] : 0.0;
+#pragma unroll
+            for (int r = 0; r < IU; ++r)
+#pragma unroll
+                for (int u = 0; u < PU; ++u) acc[u] = fma(x[r][u], c[i + r], acc[u]);
+        }
+        for (; i < m; ++i)
+#pragma unroll
+            for (int u = 0; u < PU; ++u)
+                acc[u] = fma(ok[u] ? vp[(size_t)i * ld + u * KB] : 0.0, c[i], acc[u]);
+#pragma unroll
+        for (int u = 0; u < PU; ++u) {
+            const int64_t t = tb + u * KB + threadIdx.x;
+            if (ok[u]) w[t] = w[t] - acc[u];
+        }
     }
+    // each thread reads back only the w[t] it wrote
     const int cols = norm_only ? 1 : m;
     for (int i0 = 0; i0 < cols; i0 += CB) {
         double s[CB];
-#pragma unroll
-        for (int k = 0; k < CB; ++k) s[k] = 0.0;
-        for (int64_t t = t0 + threadIdx.x; t < t1; t += KB) {
-            const double wt = w[t];
-#pragma unroll
-            for (int k = 0; k < CB; ++k) {
-                const int i = i0 + k;
-                if (i < cols) s[k] += (norm_only ? wt : V[(size_t)i * ld + t]) * wt;
-            }
-        }
+        dot_pass(s, i0, cols, norm_only ? 0 : m, ld, t0, t1, V, w);
         store_block_sums(s, red, i0, cols, nb, partial_out);
     }
 }

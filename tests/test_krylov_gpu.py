@@ -9,7 +9,8 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize('m,n', [(1, 1000), (7, 5000), (40, 123457), (300, 2049), (60, 400000)])
+@pytest.mark.parametrize('m,n', [(1, 1000), (7, 5000), (40, 123457), (300, 2049), (60, 400000),
+                                 (5, 1500001)])
 def test_cgs2_and_combine_match_torch(m, n):
     from tectosaur_amd import _native as N
     rng = np.random.default_rng(m)
