@@ -6,17 +6,30 @@ namespace tct {
 
 // out [NO][n_obs_pad] = sum over each tile's segments [seg_off[t],
 // seg_off[t + 1]) of seg_partial, in segment order.  grid = n_obs_tiles,
-// one thread per (output q, point).
+// one thread per (output q, point).  The loads of RS segments are issued
+// before their values are added (in segment order), so a tile with many
+// segments waits on memory once per RS segments, not once per segment.
 __global__ __launch_bounds__(576) void tree_reduce_kernel(int no, int n_obs_pad,
                                                           const int *__restrict__ seg_off,
                                                           const double *__restrict__ seg_partial,
                                                           double *__restrict__ out) {
+    constexpr int RS = 8;
     const int t = blockIdx.x;
     const int q = threadIdx.x / G, lane = threadIdx.x % G;
     if (q >= no) return;
     const int s0 = seg_off[t], s1 = seg_off[t + 1];
+    const size_t stride = (size_t)no * G;
+    const double *p = seg_partial + (size_t)q * G + lane;
     double v = 0.0;
-    for (int s = s0; s < s1; ++s) v += seg_partial[((size_t)s * no + q) * G + lane];
+    int s = s0;
+    for (; s + RS <= s1; s += RS) {
+        double x[RS];
+#pragma unroll
+        for (int k = 0; k < RS; ++k) x[k] = p[(size_t)(s + k) * stride];
+#pragma unroll
+        for (int k = 0; k < RS; ++k) v += x[k];
+    }
+    for (; s < s1; ++s) v += p[(size_t)s * stride];
     out[(size_t)q * n_obs_pad + t * G + lane] = v;
 }
 

@@ -65,15 +65,35 @@ __global__ __launch_bounds__(256) void bsr9_kernel(int n_block_rows,
     }
 }
 
+// One thread per row.  The row's entries are taken RC at a time: their
+// column indices and values load together, then the RC gathered x values,
+// then the products are added in entry order (one fused multiply-add each),
+// so a row waits on memory twice per RC entries rather than twice per entry.
 __global__ __launch_bounds__(256) void csr_kernel(int n_rows, const int32_t *__restrict__ indptr,
                                                   const int32_t *__restrict__ indices,
                                                   const double *__restrict__ data,
                                                   const double *__restrict__ x,
                                                   double *__restrict__ y, int accumulate) {
+    constexpr int RC = 4;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
+    const int k1 = indptr[r + 1];
+    int k = indptr[r];
     double s = 0.0;
-    for (int k = indptr[r]; k < indptr[r + 1]; ++k) s += data[k] * x[indices[k]];
+    for (; k + RC <= k1; k += RC) {
+        int c[RC];
+        double a[RC], xv[RC];
+#pragma unroll
+        for (int j = 0; j < RC; ++j) {
+            c[j] = indices[k + j];
+            a[j] = data[k + j];
+        }
+#pragma unroll
+        for (int j = 0; j < RC; ++j) xv[j] = x[c[j]];
+#pragma unroll
+        for (int j = 0; j < RC; ++j) s = fma(a[j], xv[j], s);
+    }
+    for (; k < k1; ++k) s = fma(data[k], x[indices[k]], s);
     y[r] = accumulate ? y[r] + s : s;
 }
 
