@@ -99,19 +99,34 @@ class _Active:
         self.busy = False     # a step is in flight
 
 
-def _release(a, capture_lock=None):
+# A graph capture allocates from a private memory pool of torch's caching
+# allocator.  When the graph is destroyed the pool's memory stays reserved
+# and is never handed out again until the cache is emptied (or an
+# allocation fails): ~0.3 GB per served solve, ~72 GB per 256-request
+# stream, which left later streams of one process with too little free HBM
+# to admit their solves.  A release empties the cache once this much
+# reserved memory is unallocated.
+CACHE_SLACK_BYTES = 32 << 30
+
+
+def _release(a, capture_lock=None, cache_slack=None):
     """Drop a finished solve's device objects (and with them its HIP graphs)
     on the calling thread, never during a graph capture (``capture_lock``).
     Its buffers were allocated on its build stream and return to that
     stream's pool: the build stream first waits for whatever is still queued
     on the solve's stream, so a later build cannot reuse the memory under
-    it."""
+    it.  With ``cache_slack`` (bytes), the caching allocator's cache is
+    emptied, under the same lock, when more than that is reserved but
+    unallocated (the memory pools of destroyed graphs among it)."""
     if a.build_stream is not None and a.stream is not None:
         a.build_stream.wait_stream(a.stream)
     with capture_lock or contextlib.nullcontext():
         a.op = None
         a.state = None
         a.prob = None
+        if cache_slack is not None and (torch.cuda.memory_reserved() -
+                                        torch.cuda.memory_allocated() > cache_slack):
+            torch.cuda.empty_cache()
 
 
 _PREP_LOCAL = threading.local()
@@ -183,7 +198,8 @@ def _finalize(a, now, keep_slip):
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
                  hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None,
-                 graphs=True, streams=4, switch_interval=1e-3, builders=1, batch_nbody=False):
+                 graphs=True, streams=4, switch_interval=1e-3, builders=1, batch_nbody=False,
+                 cache_slack=CACHE_SLACK_BYTES):
         # device queries from worker threads start from cached values
         from . import _native as N
         from .autotune import device_arch
@@ -195,6 +211,9 @@ class Server:
         self.restart = restart
         self.maxiter = maxiter
         self.hbm_fraction = hbm_fraction
+        # unallocated reserved memory above which a release empties the
+        # allocator's cache (None: never)
+        self.cache_slack = cache_slack
         self.orders = orders
         # mode 'tree': treecode parameters of the inner farfield
         self.tree = (tree or dict(DEFAULT_TREE)) if mode == 'tree' else None
@@ -337,6 +356,11 @@ class Server:
         recs = {}
         active = []
         used = 0
+        # the admission budget is a fraction of the free HBM: memory the
+        # allocator holds cached (earlier runs' graph pools among it) is
+        # released first so that it counts as free
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
         free0, _ = torch.cuda.mem_get_info()
         budget = self.hbm_fraction * free0
         pool = (concurrent.futures.ThreadPoolExecutor(self.prep_workers)
@@ -480,7 +504,8 @@ class Server:
                         if builder is not None:
                             # the solve's graphs are destroyed on a builder
                             # thread, never while a capture is under way
-                            builder.submit(_release, a, self.capture_lock)
+                            builder.submit(_release, a, self.capture_lock,
+                                           self.cache_slack)
                 if finished:
                     active = [a for a in active if id(a) not in finished]
         finally:

@@ -59,6 +59,26 @@ def test_release_destroys_graphs_under_the_capture_lock_after_stream_join():
     assert not lock.held()
 
 
+def test_release_empties_the_allocator_cache_beyond_the_slack(monkeypatch):
+    # destroyed graphs leave their memory pools reserved: a release empties
+    # the cache, under the capture lock, once the unallocated reserved
+    # memory exceeds the slack, and not before
+    lock = _LockProbe()
+    calls = []
+    mem = {'reserved': 0, 'allocated': 0}
+    monkeypatch.setattr(serve.torch.cuda, 'memory_reserved', lambda *a, **k: mem['reserved'])
+    monkeypatch.setattr(serve.torch.cuda, 'memory_allocated', lambda *a, **k: mem['allocated'])
+    monkeypatch.setattr(serve.torch.cuda, 'empty_cache', lambda: calls.append(lock.held()))
+    for reserved, allocated, slack, want in ((100, 10, 50, [True]), (100, 60, 50, []),
+                                             (10 ** 12, 0, None, [])):
+        mem.update(reserved=reserved, allocated=allocated)
+        calls.clear()
+        a = serve._Active(None, object(), object(), object(), 0)
+        serve._release(a, lock, slack)
+        assert calls == want, (reserved, allocated, slack)
+        assert a.op is None and not lock.held()
+
+
 def test_build_captures_under_the_capture_lock(monkeypatch):
     lock = _LockProbe()
     seen = {}
