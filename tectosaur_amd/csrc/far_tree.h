@@ -29,10 +29,11 @@
 // Layout: the tree is a binary tree over the 64-point Morton tiles of the
 // packed source set (far_p2p.hip's layout; a node covers a contiguous range
 // of tiles).  Interaction lists are per observation tile (one wave): entry
-// e >= 0 is an m2p node, e < 0 is a p2p source tile -(e + 1).  Each wave
-// walks its list with wave-uniform loads (node moments and source records
-// come through the scalar cache).  grid.y splits each list into chunks
-// whose partial sums are added in fixed order by the reduce/scatter step.
+// e >= 0 is an m2p node, e < 0 is a p2p source tile -(e + 1).  Each list
+// is split into segments of a few entries, one wave per segment, which
+// walks it with wave-uniform loads (source records through the scalar
+// cache, node moments staged in LDS); tree_reduce_kernel adds a tile's
+// segment sums in segment order.
 #include <type_traits>
 #include <utility>
 
@@ -296,10 +297,12 @@ __global__ __launch_bounds__(256) void tree_m2m_kernel(int order, int p_off, int
 // to order P + 1 at u = (c - x)/rho; Rt = c - x (physical); M: the node's
 // scaled moments [C][n_terms(P)] (wave-uniform); out accumulates NO sums.
 //
-// The moments are wave-uniform and come in through scalar loads.  They are
+// The moments are wave-uniform: tree_eval_kernel stages the node's C x TT
+// values in LDS and they are read back as broadcasts into VGPRs.  They are
 // consumed in groups of a few channels x CH terms, fenced by scheduling
-// barriers, so only ~32 of them are live in SGPRs at a time (without the
-// fences the scheduler hoists every load of the node and the SGPRs spill).
+// barriers, so only a few dozen of them are live in registers at a time
+// (without the fences the scheduler hoists every load of the node and the
+// registers spill).
 template <int KID>
 struct TreeChunk {
     static constexpr int CH = KID == TCT_RT ? 4 : 8;
