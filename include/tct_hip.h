@@ -101,9 +101,9 @@ int tct_far_pack_obs(int64_t n, const double *pts, const double *ns, int64_t n_p
 int tct_far_pack_src(int64_t n, const double *pts, const double *ns, int64_t n_pad,
                      float *rec32, double *org, double *rec64, void *stream);
 
-/* tct_far_pack_src_ex: as tct_far_pack_src with rec_words (12, or 24 for the
- * elasticRH3 record) words per source record; words past the normal are
- * zeroed. */
+/* tct_far_pack_src_ex: as tct_far_pack_src with the record size given
+ * (rec_words must be TCT_REC: every kernel's record, elasticRH3's
+ * [y, G(9)] included, is TCT_REC words); words past the normal are zeroed. */
 int tct_far_pack_src_ex(int64_t n, const double *pts, const double *ns, int64_t n_pad,
                         int rec_words, float *rec32, double *org, double *rec64, void *stream);
 

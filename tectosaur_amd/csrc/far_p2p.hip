@@ -30,7 +30,7 @@ namespace tct {
 // Words of a source record that the pair body of KID reads.
 template <int KID>
 constexpr int used_words() {
-    return KID == TCT_RH ? 21 : KID == TCT_RT ? 12 : KID == TCT_H ? 11 : KID == TCT_T ? 10 : 9;
+    return KID == TCT_RH ? 12 : KID == TCT_RT ? 12 : KID == TCT_H ? 11 : KID == TCT_T ? 10 : 9;
 }
 // Source records of a segment-table launch are read through the constant
 // address space: the kernel never writes them, and a uniform load from there
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
         } else if (__any(close)) {
             // the same pipelined loop with the r2 test (see below)
             constexpr int NW = used_words<KID>();
-            constexpr int U = NW > 12 ? 1 : 2;
+            constexpr int U = (NW > 12 || KID == TCT_RH) ? 1 : 2;
             float qa[U][NW], qb[U][NW];
             load_recs<U, NW, RW>(qa, r);
 #pragma unroll 1
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void far_p2p_f32_kernel(
             // for one waits for all; fence_recs places that wait before the
             // next loads are issued and the pairs that need it after it.
             constexpr int NW = used_words<KID>();
-            constexpr int U = NW > 12 ? 1 : 2;
+            constexpr int U = (NW > 12 || KID == TCT_RH) ? 1 : 2;
             float qa[U][NW], qb[U][NW];
             load_recs<U, NW, RW>(qa, r);
 #pragma unroll 1
@@ -584,14 +584,11 @@ __global__ void gather_curl_kernel(int64_t n_pts, int64_t n_pad, int nq,
     } else if constexpr (KID == TCT_RA) {
         for (int d = 0; d < 3; ++d) r[6 + d] = (T)(f * in[d]);
     } else {
+        // G only: the kernels apply P(G) = CsRH1 tr(G) I + CsRH2 G^T - 2 G
+        // to their sums of G / r
         const double g = f * c.CsRH0;
-        const double B = c.CsRH1 * (S[0][0] + S[1][1] + S[2][2]);
         for (int d = 0; d < 3; ++d)
-            for (int mm = 0; mm < 3; ++mm) {
-                r[3 + d * 3 + mm] = (T)(g * S[d][mm]);
-                r[12 + d * 3 + mm] =
-                    (T)(g * ((d == mm ? B : 0.0) + c.CsRH2 * S[mm][d] - 2.0 * S[d][mm]));
-            }
+            for (int mm = 0; mm < 3; ++mm) r[3 + d * 3 + mm] = (T)(g * S[d][mm]);
     }
 }
 
@@ -842,7 +839,7 @@ extern "C" int tct_far_pack_src_ex(int64_t n, const double *pts, const double *n
                                    void *stream) {
     TCT_CHECK_ARG(n > 0 && n_pad >= n && n_pad % G == 0, "pack_src: bad sizes n=%lld n_pad=%lld",
                   (long long)n, (long long)n_pad);
-    TCT_CHECK_ARG(rec_words == 12 || rec_words == 24, "pack_src: rec_words %d", rec_words);
+    TCT_CHECK_ARG(rec_words == R, "pack_src: rec_words %d (records are %d words)", rec_words, R);
     TCT_CHECK_ARG(pts && ns && rec32 && org && rec64, "pack_src: null pointer");
     hipLaunchKernelGGL(pack_src_kernel, dim3((unsigned)(n_pad / G)), dim3(G), 0,
                        as_stream(stream), n, pts, ns, n_pad, rec_words, rec32, org, rec64);

@@ -34,17 +34,20 @@ __device__ __forceinline__ double rsq_masked(double r2, double eps2) {
 }
 
 template <int KID>
-constexpr int rec_words() { return KID == TCT_RH ? 24 : 12; }
+constexpr int rec_words() { return R; }
 template <int KID>
 constexpr int n_out() { return (KID == TCT_RA || KID == TCT_RH) ? 9 : 3; }
 template <int KID>
 constexpr bool needs_obs_normal() { return KID == TCT_A || KID == TCT_H || KID == TCT_RA; }
-// Terms whose vector is the observation normal m are summed as per-lane
-// scalars (am) and multiplied by m once per source tile (fold_mterm), not
-// per pair: A has one (-CsT0/CsT1 (D.s)/r^3), H two (e0 / r^3 and
-// 3 CsH1 (n.D)(s.D)/r^5).
+// Sums that are mapped to the outputs once per source tile (fold_mterm)
+// rather than per pair, as per-lane scalars (am): terms whose vector is the
+// observation normal m, multiplied by m once per tile -- A has one
+// (-CsT0/CsT1 (D.s)/r^3), H two (e0 / r^3 and 3 CsH1 (n.D)(s.D)/r^5) -- and
+// RH's Phi = sum G / r (9), to which the linear map P(G) = CsRH1 tr(G) I +
+// CsRH2 G^T - 2 G is applied once per tile, so the source record carries G
+// only.
 template <int KID>
-constexpr int n_mterm() { return KID == TCT_H ? 2 : (KID == TCT_A ? 1 : 0); }
+constexpr int n_mterm() { return KID == TCT_H ? 2 : (KID == TCT_A ? 1 : (KID == TCT_RH ? 9 : 0)); }
 template <int KID>
 constexpr int n_mterm_arr() { return n_mterm<KID>() > 0 ? n_mterm<KID>() : 1; }
 
@@ -67,10 +70,9 @@ __device__ __forceinline__ T dot3(T a0, T a1, T a2, T b0, T b1, T b2) {
 //   RT:      q = [y n s e], s = CsRT2 w J in(q), e = CsRT1 w J a with
 //            a_t = sum_{d,I} eps_{I t d} S_{dI} (S the source surface curl);
 //   RA:      q = [y - s -], s = w J in(q);
-//   RH:      q = [y G(9) P(9) -], G = w J CsRH0 S,
-//            P = w J CsRH0 (CsRH1 tr(S) I + CsRH2 S^T - 2 S).
+//   RH:      q = [y G(9)], G = w J CsRH0 S (G[j][m] at q[3 + 3 j + m]).
 // The FP32 N-body's pipelined loop copies only the leading words each body
-// reads (used_words in far_p2p.hip: U/A/RA 9, T 10, H 11, RT 12, RH 21); a
+// reads (used_words in far_p2p.hip: U/A/RA 9, T 10, H 11, RT 12, RH 12); a
 // body that reads a later word must raise that count.
 // Every fused multiply-add is written out (fma_t) and the compiler may form
 // no other (contract(off)): left to itself it fused a + b*c + d*e on one
@@ -199,7 +201,8 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
         acc[7] = fma_t(mD, s1, acc[7]);
         acc[8] = fma_t(mD, s2, acc[8]);
     } else {
-        // RH: M[k][m] += P[k][m] / r - 2 D_k (D^T G)_m / r^3
+        // RH: M[k][m] += P(G)[k][m] / r - 2 D_k (D^T G)_m / r^3, the first
+        // term as Phi[k][m] += G[k][m] / r (am), mapped by P once per tile
         const T Dk[3] = {D0, D1, D2};
         const T qd = (T(-2) * invr) * w;
         T u[3];
@@ -210,7 +213,7 @@ __device__ __forceinline__ void pair_eval(const T x[3], const T m[3], const T *_
             const T dq = qd * Dk[kk];
 #pragma unroll
             for (int mm = 0; mm < 3; ++mm) {
-                acc[kk * 3 + mm] = fma_t(invr, q[12 + kk * 3 + mm], acc[kk * 3 + mm]);
+                am[kk * 3 + mm] = fma_t(invr, q[3 + kk * 3 + mm], am[kk * 3 + mm]);
                 acc[kk * 3 + mm] = fma_t(dq, u[mm], acc[kk * 3 + mm]);
             }
         }
@@ -233,6 +236,21 @@ __device__ __forceinline__ void fold_mterm(const T m[3], const KConst<T> &k, T *
         acc[1] = fma_t(m[1], f, acc[1]);
         acc[2] = fma_t(m[2], f, acc[2]);
         am[0] = T(0);
+    } else if constexpr (KID == TCT_RH) {
+        // acc += P(Phi) = CsRH1 tr(Phi) I + CsRH2 Phi^T - 2 Phi
+        // (k.c0 = CsRH1, k.c1 = CsRH2)
+        const T tr = (am[0] + am[4]) + am[8];
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm) {
+                T a = fma_t(k.c1, am[mm * 3 + kk], acc[kk * 3 + mm]);
+                a = fma_t(T(-2), am[kk * 3 + mm], a);
+                if (kk == mm) a = fma_t(k.c0, tr, a);
+                acc[kk * 3 + mm] = a;
+            }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) am[q] = T(0);
     }
 }
 
@@ -254,8 +272,8 @@ inline KConst<T> kernel_consts(int kid, const double *params, double eps2) {
         break;
     case TCT_RT: k.c0 = (T)(c.CsRT0 / c.CsRT1); break;
     case TCT_RH:
-        // used by the treecode's RH expansion only (the point form folds
-        // every constant into the source record)
+        // the map P(G) = CsRH1 tr(G) I + CsRH2 G^T - 2 G, applied per tile
+        // (point form) and per node (treecode expansion)
         k.c0 = (T)c.CsRH1;
         k.c1 = (T)c.CsRH2;
         break;

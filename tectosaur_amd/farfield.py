@@ -94,7 +94,7 @@ class FarfieldEngine:
         K = KERNELS[K_name]
         self.K = K
         # record words per source point and sums per observation point
-        self.rec_words = 24 if K_name == 'elasticRH3' else REC
+        self.rec_words = REC
         self.n_out = 9 if K_name in ('elasticRA3', 'elasticRH3') else 3
         self.kid = K.kid
         self.params = N.params_array(params)
