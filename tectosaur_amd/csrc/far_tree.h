@@ -31,9 +31,9 @@
 // of tiles).  Interaction lists are per observation tile (one wave): entry
 // e >= 0 is an m2p node, e < 0 is a p2p source tile -(e + 1).  Each list
 // is split into segments of a few entries, one wave per segment, which
-// walks it with wave-uniform loads (source records through the scalar
-// cache, node moments staged in LDS); tree_reduce_kernel adds a tile's
-// segment sums in segment order.
+// walks it with wave-uniform reads (FP32 source tiles and node moments
+// staged in LDS, FP64 source records through the scalar cache);
+// tree_reduce_kernel adds a tile's segment sums in segment order.
 #include <type_traits>
 #include <utility>
 
@@ -440,10 +440,10 @@ __device__ __forceinline__ void m2p_eval(const T *a, const T Rt[3], const T *__r
 // and tree_reduce_kernel adds a tile's segments in order.
 // FP32 (ABS = false): obs = loc32 [6][n_obs_pad] local to obs_org per tile,
 // records local to src_org per tile.  FP64 (ABS = true): obs = soa64
-// absolute, records absolute.  p2p source records are read with scalar
-// loads (wave-uniform, as in far_p2p.hip); an m2p node's moments (C x TT
-// values, too many for SGPRs) are copied by the wave into LDS and read back
-// as broadcasts.
+// absolute, records absolute.  A p2p tile's source records are copied by
+// the wave into LDS and read back as broadcasts (FP32; FP64 reads them with
+// wave-uniform scalar loads); an m2p node's moments (C x TT values, too many
+// for SGPRs) are copied into LDS the same way.
 template <int KID, int P, typename T, bool ABS>
 __global__ __launch_bounds__(64) void tree_eval_kernel(
     int n_obs_pad, const T *__restrict__ obs, const double *__restrict__ obs_org,
@@ -456,7 +456,24 @@ __global__ __launch_bounds__(64) void tree_eval_kernel(
     constexpr int C = TreeK<KID>::C;
     constexpr int TT = n_terms(P);
     constexpr int NM = C * TT;
+    // FP32: a p2p tile's 64 source records are staged in LDS (lane l copies
+    // 16-byte words l, l + 64, ... of the tile) and read back with broadcast
+    // LDS reads, so the pair body's operands are VGPRs, not the SGPRs of
+    // scalar loads (an instruction that reads an SGPR is never dual-issued);
+    // the kernel constants are kept in VGPRs for the same reason
+    constexpr bool STAGE = !ABS && RW % 4 == 0;
+    typedef float tf4 __attribute__((ext_vector_type(4)));
     __shared__ T smom[NM];
+    __shared__ tf4 srec[STAGE ? G * RW / 4 : 1];
+    KConst<T> kv = k;
+    if constexpr (STAGE) {
+        asm volatile("" : "+v"(kv.c0));
+        asm volatile("" : "+v"(kv.c1));
+        asm volatile("" : "+v"(kv.c2));
+        asm volatile("" : "+v"(kv.c3));
+        asm volatile("" : "+v"(kv.c4));
+        asm volatile("" : "+v"(kv.eps2));
+    }
     const int sg = blockIdx.x;
     const int tile = seg[3 * sg];
     const int lane = (int)threadIdx.x;
@@ -489,9 +506,30 @@ __global__ __launch_bounds__(64) void tree_eval_kernel(
                 xs[d] = ABS ? x[d] : x[d] - (T)(src_org[3 * t + d] - oo[d]);
             const T *r = rec + (size_t)t * G * RW;
             T am[n_mterm_arr<KID>()] = {};
+            if constexpr (STAGE) {
+                const tf4 *src = (const tf4 *)r;
+#pragma unroll
+                for (int v = 0; v < RW / 4; ++v) srec[v * G + lane] = src[v * G + lane];
+                __syncthreads();
 #pragma unroll 2
-            for (int j = 0; j < G; ++j) pair_eval<KID, T>(xs, mn, r + j * RW, k, acc, am);
-            fold_mterm<KID, T>(mn, k, acc, am);
+                for (int j = 0; j < G; ++j) {
+                    T q[RW];
+#pragma unroll
+                    for (int v = 0; v < RW / 4; ++v) {
+                        const tf4 f = srec[j * (RW / 4) + v];
+                        q[4 * v] = f.x;
+                        q[4 * v + 1] = f.y;
+                        q[4 * v + 2] = f.z;
+                        q[4 * v + 3] = f.w;
+                    }
+                    pair_eval<KID, T>(xs, mn, q, kv, acc, am);
+                }
+                __syncthreads();
+            } else {
+#pragma unroll 2
+                for (int j = 0; j < G; ++j) pair_eval<KID, T>(xs, mn, r + j * RW, k, acc, am);
+            }
+            fold_mterm<KID, T>(mn, kv, acc, am);
         } else {
             const T *Mg = moments + (size_t)code * NM;
             constexpr int NL = (NM + G - 1) / G;
