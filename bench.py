@@ -420,7 +420,15 @@ def large_solve_bench(mode='tree', tol=1e-8):
     def progress(rel):
         print('# large solve: residual %.3e at %.1f s' % (rel, time.perf_counter() - t0),
               flush=True)
-    u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=tol, mode=mode, callback=progress)
+
+    def step(it, est):
+        # with the direct inner operator a cycle takes minutes: report its
+        # iterations as they go
+        if mode == 'mixed' and it % 16 == 0:
+            print('# large solve: iteration %d, estimated residual %.2e at %.1f s' % (
+                it, est, time.perf_counter() - t0), flush=True)
+    u, res = solve_constrained(prob.H, prob.cm, prob.rhs, tol=tol, mode=mode, callback=progress,
+                               step_callback=step)
     torch.cuda.synchronize()
     t_solve = time.perf_counter() - t0
     inner = {'tree': 'FP32 treecode (order %d, mac %g) + FP32 nearfield blocks' % (
@@ -577,6 +585,9 @@ def main():
     ap.add_argument('--no-krylov', action='store_true')
     ap.add_argument('--large-solve', action='store_true',
                     help='only the single large solve (655,360 triangles; minutes)')
+    ap.add_argument('--large-solve-mode', default='tree', choices=('tree', 'mixed'),
+                    help="inner operator of the large solve: 'tree' (FP32 treecode) or "
+                         "'mixed' (the direct FP32 farfield; ~15 min)")
     ap.add_argument('--valu-calibration', action='store_true',
                     help='time the VALU issue-rate calibration kernels first')
     ap.add_argument('--copy-calibration', action='store_true',
@@ -591,7 +602,8 @@ def main():
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
     torch.cuda.set_device(0)
     if args.large_solve:
-        print(json.dumps({'large_solve': large_solve_bench()}), flush=True)
+        print(json.dumps({'large_solve': large_solve_bench(mode=args.large_solve_mode)}),
+              flush=True)
         return
     calib = copy_calibration() if args.copy_calibration else None
     valu_cal = valu_calibration() if args.valu_calibration else None

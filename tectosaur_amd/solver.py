@@ -228,7 +228,7 @@ class GmresState:
     def __init__(self, apply, b, tol=1e-8, restart=200, maxiter=2000, inner_prec='f64',
                  outer_prec='f64', inner_reduction=None, x0=None, callback=None,
                  coarse_prec=None, coarse_above=3e-5, update_prec=None, update_below=1e-3,
-                 operator=None):
+                 operator=None, step_callback=None):
         self.apply = apply
         self.b = b
         dev = b.device
@@ -258,6 +258,9 @@ class GmresState:
         self._expect = float('inf')
         self._res_prec = outer_prec
         self.callback = callback
+        # called with (iterations, estimated relative residual) after every
+        # Arnoldi step
+        self.step_callback = step_callback
         self.x = torch.zeros(n, dtype=torch.float64, device=dev) if x0 is None else x0.clone()
         self._fresh = x0 is None
         self.V = torch.empty((self.m + 1, n), dtype=torch.float64, device=dev)
@@ -490,6 +493,8 @@ class GmresState:
         self.g[j] = c * gj
         self.g.append(-sv * gj)
         est = abs(self.g[j + 1])
+        if self.step_callback is not None:
+            self.step_callback(self.it, est / self.bnorm)
         k = j + 1
         end = (not math.isfinite(est) or est <= self.cycle_target or hnext == 0.0 or
                k >= self.m or self.it >= self.maxiter)
@@ -575,15 +580,16 @@ UPDATE_PREC = {'f64': None, 'mixed': 'f32', 'tree': 'f32'}
 
 
 def make_state(op, b, mode='mixed', tol=1e-8, restart=200, maxiter=2000, callback=None,
-               inner_reduction=None):
+               inner_reduction=None, step_callback=None):
     inner, outer, red, coarse = MODES[mode]
     return GmresState(op.apply, b, tol=tol, restart=restart, maxiter=maxiter, inner_prec=inner,
                       outer_prec=outer, inner_reduction=inner_reduction or red, callback=callback,
-                      coarse_prec=coarse, update_prec=UPDATE_PREC[mode], operator=op)
+                      coarse_prec=coarse, update_prec=UPDATE_PREC[mode], operator=op,
+                      step_callback=step_callback)
 
 
 def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxiter=6000,
-                      callback=None):
+                      callback=None, step_callback=None):
     """Solve cm^T A cm x = cm^T rhs; return (u = cm x (device), GmresResult).
 
     mode 'f64': the plain FP64 path (FP64 farfield everywhere).
@@ -595,7 +601,8 @@ def solve_constrained(A, cm, rhs_dev, tol=1e-8, mode='mixed', restart=1500, maxi
         raise ValueError('mode must be one of %s' % sorted(MODES))
     op = ConstrainedOperator(A, cm)
     b = op.cmT.dot_dev(rhs_dev)
-    res = make_state(op, b, mode, tol, restart, maxiter, callback).run()
+    res = make_state(op, b, mode, tol, restart, maxiter, callback,
+                     step_callback=step_callback).run()
     u = op.cm.dot_dev(res.x)
     res.n_matvec = dict(op.n_matvec)
     return u, res

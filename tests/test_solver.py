@@ -139,3 +139,23 @@ def test_gmres_reports_nonconvergence():
     b = torch.as_tensor(rng.normal(size=n))
     r = gmres(dense_apply(A), b, tol=1e-14, restart=3, maxiter=1)
     assert not r.converged and r.info != 0 and r.iterations == 1
+
+
+def test_step_callback_sees_every_arnoldi_step():
+    # the per-step callback gets the running iteration count and the cycle's
+    # residual estimate (monotone within a cycle, relative to ||b||)
+    from tectosaur_amd.solver import GmresState
+    rng = np.random.default_rng(5)
+    n = 50
+    A = np.eye(n) * 3 + rng.normal(size=(n, n)) / np.sqrt(n)
+    b = rng.normal(size=n)
+    seen = []
+    st = GmresState(dense_apply(A), torch.as_tensor(b), tol=1e-10, restart=8, maxiter=500,
+                    step_callback=lambda it, est: seen.append((it, est)))
+    res = st.run()
+    assert res.converged
+    assert [it for it, _ in seen] == list(range(1, res.iterations + 1))
+    for (i0, e0), (i1, e1) in zip(seen, seen[1:]):
+        if i1 % 8 != 1:     # same cycle: GMRES residual estimates do not grow
+            assert e1 <= e0 * (1 + 1e-12)
+    assert seen[-1][1] <= 1e-10 * 1.01
