@@ -75,7 +75,7 @@ def profiled_traffic(kernel, tile):
     return None, 'no profile of this kernel at this launch shape'
 
 
-def cfg2_mesh():
+def two_surface_mesh():
     from tectosaur_amd import mesh
     return mesh.two_surface_mesh()
 
@@ -608,7 +608,7 @@ def main():
     calib = copy_calibration() if args.copy_calibration else None
     valu_cal = valu_calibration() if args.valu_calibration else None
     t_setup = time.perf_counter()
-    pts, tris = cfg2_mesh()
+    pts, tris = two_surface_mesh()
     n = tris.shape[0]
     v = np.random.default_rng(0).standard_normal(n * 9)
     v_dev = torch.as_tensor(v, dtype=torch.float64, device='cuda')

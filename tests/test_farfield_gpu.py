@@ -124,16 +124,16 @@ def test_tri_tri_overlapping_subsets(oracle):
         assert rel_l2(out, ref) <= TOL[prec]
 
 
-def cfg1_mesh():
+def fault_2k_mesh():
     # vertical fault 2 km x 2 km, 32 x 32 squares -> 2048 triangles
     return mesh.make_rect(33, 33, [[-1000, 0, 0], [1000, 0, 0], [1000, 0, -2000],
                                    [-1000, 0, -2000]])
 
 
 @pytest.mark.parametrize('prec', ['f32', 'f64'])
-def test_cfg1_u_farfield(oracle, prec):
+def test_fault_2k_u_farfield(oracle, prec):
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
-    pts, tris = cfg1_mesh()
+    pts, tris = fault_2k_mesh()
     n = tris.shape[0]
     ft = np.float32 if prec == 'f32' else np.float64
     op = TriToTriDirectFarfieldOp(1, 'elasticU3', [3e10, 0.25], pts, tris, ft, np.arange(n),
@@ -154,7 +154,7 @@ def test_tile_shapes_agree(cfg):
     # order with the same chunks
     from tectosaur_amd import _native as N
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
-    pts, tris = cfg1_mesh()
+    pts, tris = fault_2k_mesh()
     n = tris.shape[0]
     v = np.random.default_rng(9).normal(size=n * 9)
     base = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, np.float64,
@@ -174,7 +174,7 @@ def test_tile_shapes_agree(cfg):
 def test_xcd_order_needs_chunks_multiple_of_8():
     from tectosaur_amd import _native as N
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
-    pts, tris = cfg1_mesh()
+    pts, tris = fault_2k_mesh()
     n = tris.shape[0]
     op = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, np.float32,
                                   np.arange(n), np.arange(n), tile_cfg=N.TileCfg(256, 1, 12, 2))
@@ -184,7 +184,7 @@ def test_xcd_order_needs_chunks_multiple_of_8():
 
 def test_deterministic():
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
-    pts, tris = cfg1_mesh()
+    pts, tris = fault_2k_mesh()
     n = tris.shape[0]
     op = TriToTriDirectFarfieldOp(1, 'elasticH3', [3e10, 0.25], pts, tris, np.float32,
                                   np.arange(n), np.arange(n))
@@ -247,7 +247,7 @@ def test_tiles_without_the_r2_test_change_nothing(K):
     # records staged through LDS (bit 2), with and without the test
     from tectosaur_amd import _native as N
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
-    pts, tris = cfg1_mesh()
+    pts, tris = fault_2k_mesh()
     n = tris.shape[0]
     for obs, src in ((np.arange(n), np.arange(n)), (np.arange(n // 2 + 300),
                                                     np.arange(n // 2 - 300, n))):
@@ -271,7 +271,7 @@ def test_tile_skip_is_safe_for_any_eps2(K):
     # in one packing has r2 == 0 exactly)
     from tectosaur_amd import _native as N
     from tectosaur_amd.sparse_farfield_op import TriToTriDirectFarfieldOp
-    pts, tris = cfg1_mesh()
+    pts, tris = fault_2k_mesh()
     n = tris.shape[0]
     outs = []
     for flags in (0, 1):

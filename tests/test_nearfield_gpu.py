@@ -230,7 +230,7 @@ def test_bsr_empty_rows_and_matrix():
 
 
 @pytest.mark.parametrize('Kn,Kf', [('elasticRH3', 'elasticH3'), ('elasticRT3', 'elasticT3')])
-def test_cfg3_icosphere_l5_subset(oracle, Kn, Kf):
+def test_icosphere_l5_sampled_pairs(oracle, Kn, Kf):
     # icosphere level 5 (20,480 triangles), all rules n = 8, near threshold 3:
     # every class assembled on the device; a seeded sample of pairs of each
     # class is integrated by the oracle and compared elementwise, both the
