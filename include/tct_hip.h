@@ -275,11 +275,12 @@ int tct_tree_reduce(int kernel, int n_obs_tiles, int n_obs_pad, const int *seg_o
  * quadrature points; 0 picks from nq and n_pairs (the fewest lanes that
  * still fill the device; for elasticRH3 with nq >= 512, where that would be
  * several lanes, one lane per pair and rule chunk instead); -C (elasticRH3,
- * C <= nq, and 1 <= C <= 11 or C a multiple of 4 up to 44) runs one lane per
- * pair and each of C rule chunks, their sums added in chunk order (out rows
- * serve as the chunks' workspace, 11 slots of 7 values; beyond 11 chunks
- * the four waves of a workgroup take four consecutive chunks of the same 64
- * pairs and add them in LDS before they fill a slot). */
+ * C <= nq, and 1 <= C <= 11, C a multiple of 4 up to 44 or a multiple of 8
+ * up to 88) runs one lane per pair and each of C rule chunks, their sums
+ * added in chunk order (out rows serve as the chunks' workspace, 11 slots
+ * of 7 values; beyond 11 chunks the four or eight waves of a workgroup take
+ * as many consecutive chunks of the same 64 pairs and add them in LDS
+ * before they fill a slot). */
 int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, const double *qw,
                         const double *pts, const int32_t *tris, int64_t n_pairs,
                         const int32_t *pairs, int pair_width, int check0,

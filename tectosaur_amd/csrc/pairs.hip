@@ -371,9 +371,9 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
 
 // RH3 pairs, one lane per (pair, rule chunk): grid (pair blocks, C slots).
 // With S = 1 a workgroup holds 256 pairs and slot ch is rule chunk ch of C,
-// points [nq ch / C, nq (ch + 1) / C).  With S = 4 a workgroup holds 64
-// pairs, its wave w takes chunk S ch + w of S C, and the waves' sums are
-// added in wave order through LDS, so a slot holds four chunks.  The rule is
+// points [nq ch / C, nq (ch + 1) / C).  With S = 4 or 8 a workgroup holds
+// 64 pairs in S waves, wave w takes chunk S ch + w of S C, and the waves'
+// sums are added in wave order through LDS, so a slot holds S chunks.  The rule is
 // read through wave-uniform scalar loads as with one lane per pair, so pairs
 // too few to fill the device one lane each still do.  The 7 accumulators of
 // slot ch go to the pair's own output row, out[pair][7 ch .. 7 ch + 7)
@@ -381,14 +381,18 @@ __global__ __launch_bounds__(256) void pairs_kernel(int nq, const double *__rest
 // the 81 values.
 constexpr int RH_MAX_CHUNKS = 81 / 7;   // chunk accumulators that fit in an output row
 constexpr int RH_WAVE_SPLIT = 4;        // chunks per slot when the waves split one
+constexpr int RH_WAVE_SPLIT_MAX = 8;    // ... and with eight waves per workgroup
+
+// threads per workgroup of the chunk kernel
+constexpr int rh_chunk_block(int S) { return S > 4 ? 64 * S : 256; }
 
 template <int CK, int S>
-__global__ __launch_bounds__(256) void pairs_rh_chunk_kernel(
+__global__ __launch_bounds__(rh_chunk_block(S)) void pairs_rh_chunk_kernel(
     int nq, const double *__restrict__ qx, const double *__restrict__ qw,
     const double *__restrict__ pts, const int32_t *__restrict__ tris, int n_pairs,
     const int32_t *__restrict__ pairs, int pair_width, ElasticConsts c, double *__restrict__ out) {
     constexpr int NA = n_acc<TCT_RH>();
-    constexpr int PB = 256 / S;   // pairs per workgroup, a multiple of 64
+    constexpr int PB = rh_chunk_block(S) / S;   // pairs per workgroup, a multiple of 64
     static_assert(PB % 64 == 0, "a wave must not straddle two chunks");
     const int lane = (int)threadIdx.x % PB;
     const int w = S > 1 ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / PB) : 0;
@@ -528,11 +532,14 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
     int chunks = 0;   // RH3 rule chunks (one lane per pair and chunk)
     if (gl < 0) {
         const int64_t cmax = (int64_t)RH_WAVE_SPLIT * RH_MAX_CHUNKS;
+        const int64_t cmax8 = (int64_t)RH_WAVE_SPLIT_MAX * RH_MAX_CHUNKS;
         TCT_CHECK_ARG(kernel == TCT_RH && -(int64_t)gl <= nq &&
-                          (-gl <= RH_MAX_CHUNKS || (-gl <= cmax && -gl % RH_WAVE_SPLIT == 0)),
-                      "pairs: %d rule chunks (RH3 only, at most nq, and at most %d or a "
-                      "multiple of %d up to %d)",
-                      -gl, RH_MAX_CHUNKS, RH_WAVE_SPLIT, (int)cmax);
+                          (-gl <= RH_MAX_CHUNKS || (-gl <= cmax && -gl % RH_WAVE_SPLIT == 0) ||
+                           (-gl <= cmax8 && -gl % RH_WAVE_SPLIT_MAX == 0)),
+                      "pairs: %d rule chunks (RH3 only, at most nq, and at most %d, a "
+                      "multiple of %d up to %d or of %d up to %d)",
+                      -gl, RH_MAX_CHUNKS, RH_WAVE_SPLIT, (int)cmax, RH_WAVE_SPLIT_MAX,
+                      (int)cmax8);
         chunks = -gl;
         gl = 1;
     } else if (gl == 0) {
@@ -553,15 +560,22 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
             }
         // RH3 with a large rule: one lane per pair and rule chunk instead of
         // several lanes per pair, enough chunks for the wavefronts the lane
-        // choice above asks for: up to RH_MAX_CHUNKS one per slot, beyond
-        // that RH_WAVE_SPLIT per slot (at most RH_MAX_CHUNKS slots)
+        // choice above asks for: up to RH_MAX_CHUNKS one per slot, then
+        // RH_WAVE_SPLIT per slot, then RH_WAVE_SPLIT_MAX per slot (at most
+        // RH_MAX_CHUNKS slots; nq >= 512 exceeds every count)
         if (kernel == TCT_RH && gl > 1 && nq >= 512) {
             const int64_t want = ceil_div(64LL * min_waves, n_pairs);
             if (want <= RH_MAX_CHUNKS) {
                 chunks = (int)(want < 2 ? 2 : want);
+            } else if (want <= (int64_t)RH_WAVE_SPLIT * RH_MAX_CHUNKS) {
+                chunks = RH_WAVE_SPLIT * (int)ceil_div(want, (int64_t)RH_WAVE_SPLIT);
             } else {
-                const int64_t slots = ceil_div(want, (int64_t)RH_WAVE_SPLIT);
-                chunks = RH_WAVE_SPLIT * (int)(slots < RH_MAX_CHUNKS ? slots : RH_MAX_CHUNKS);
+                // at least 8 slots: on the icosphere's 20,480 coincident
+                // pairs 64 chunks measured 0.585 of FP64 peak against 0.57
+                // with 56 and 0.576 with 88
+                int64_t slots = ceil_div(want, (int64_t)RH_WAVE_SPLIT_MAX);
+                slots = slots < 8 ? 8 : (slots > RH_MAX_CHUNKS ? RH_MAX_CHUNKS : slots);
+                chunks = RH_WAVE_SPLIT_MAX * (int)slots;
             }
             gl = 1;
         }
@@ -572,17 +586,26 @@ extern "C" int tct_pairs_integrate(int kernel, int64_t nq, const double *qx, con
     hipStream_t st = as_stream(stream);
     const int np = (int)n_pairs, q = (int)nq;
     if (chunks > 0) {
-        const bool split = chunks > RH_MAX_CHUNKS;
-        const int slots = split ? chunks / RH_WAVE_SPLIT : chunks;
-        const dim3 grid((unsigned)ceil_div(n_pairs, (int64_t)(split ? 256 / RH_WAVE_SPLIT : 256)),
-                        (unsigned)slots);
-#define TCT_RH_CHUNKS(CK)                                                                     \
-    if (split)                                                                                \
-        hipLaunchKernelGGL((pairs_rh_chunk_kernel<CK, RH_WAVE_SPLIT>), grid, dim3(256), 0, st, \
-                           q, qx, qw, pts, tris, np, pairs, pair_width, c, out);              \
-    else                                                                                      \
-        hipLaunchKernelGGL((pairs_rh_chunk_kernel<CK, 1>), grid, dim3(256), 0, st, q, qx, qw, \
-                           pts, tris, np, pairs, pair_width, c, out)
+        // waves per slot: 1 up to 11 chunks, then 4 (up to 44), then 8
+        const int S = chunks <= RH_MAX_CHUNKS ? 1
+                      : (chunks <= RH_WAVE_SPLIT * RH_MAX_CHUNKS && chunks % RH_WAVE_SPLIT == 0)
+                          ? RH_WAVE_SPLIT
+                          : RH_WAVE_SPLIT_MAX;
+        const int slots = chunks / S;
+        const int pb = S == 1 ? 256 : 64;   // pairs per workgroup
+        const dim3 grid((unsigned)ceil_div(n_pairs, (int64_t)pb), (unsigned)slots);
+#define TCT_RH_CHUNKS(CK)                                                                      \
+    if (S == RH_WAVE_SPLIT_MAX)                                                                \
+        hipLaunchKernelGGL((pairs_rh_chunk_kernel<CK, RH_WAVE_SPLIT_MAX>), grid,               \
+                           dim3(rh_chunk_block(RH_WAVE_SPLIT_MAX)), 0, st, q, qx, qw, pts,     \
+                           tris, np, pairs, pair_width, c, out);                               \
+    else if (S == RH_WAVE_SPLIT)                                                               \
+        hipLaunchKernelGGL((pairs_rh_chunk_kernel<CK, RH_WAVE_SPLIT>), grid,                   \
+                           dim3(rh_chunk_block(RH_WAVE_SPLIT)), 0, st, q, qx, qw, pts, tris,   \
+                           np, pairs, pair_width, c, out);                                     \
+    else                                                                                       \
+        hipLaunchKernelGGL((pairs_rh_chunk_kernel<CK, 1>), grid, dim3(rh_chunk_block(1)), 0,   \
+                           st, q, qx, qw, pts, tris, np, pairs, pair_width, c, out)
         if (check0 == CK_FULL)
             TCT_RH_CHUNKS(CK_FULL);
         else if (check0 == CK_R2)

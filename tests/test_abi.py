@@ -54,9 +54,10 @@ def test_argument_errors_are_reported_without_a_device():
                            ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf), eps2, ext,
                            ctypes.byref(ok), ctypes.byref(buf), None)
         assert rc < 0 and (b'eps2' in L.tct_last_error() or b'extent' in L.tct_last_error())
-    # RH3 rule chunks: at most 11, or a multiple of 4 up to 44, and at most nq
+    # RH3 rule chunks: at most 11, a multiple of 4 up to 44 or of 8 up to 88,
+    # and at most nq
     ip = ctypes.c_int32(0)
-    for chunks, nq in ((14, 400), (48, 400), (13, 400), (16, 8)):
+    for chunks, nq in ((14, 400), (52, 400), (96, 400), (13, 400), (16, 8)):
         rc = L.tct_pairs_integrate(6, nq, ctypes.byref(buf), ctypes.byref(buf), ctypes.byref(buf),
                                    ctypes.byref(ip), 1, ctypes.byref(ip), 2, 0,
                                    ctypes.byref(buf), -chunks, ctypes.byref(buf), None)

@@ -85,13 +85,14 @@ def test_lanes_per_pair_agree(oracle, lanes, K):
     assert ok, err
 
 
-@pytest.mark.parametrize('chunks', [1, 2, 5, 11, 12, 16, 44])
+@pytest.mark.parametrize('chunks', [1, 2, 5, 11, 12, 16, 44, 48, 88])
 def test_rh_rule_chunks_vs_oracle(oracle, chunks):
     # RH3 one lane per pair and rule chunk (lanes_per_pair = -chunks): the
     # chunks' sums added in order reproduce the integrals, for the singular
     # classes (flipped sources included) and a rule that does not split
-    # evenly; beyond 11 chunks four waves of a workgroup take four chunks of
-    # the same pairs and add them in LDS (pair counts not multiples of 64)
+    # evenly; beyond 11 chunks four (up to 44) or eight waves of a workgroup
+    # take as many chunks of the same pairs and add them in LDS (pair counts
+    # not multiples of 64)
     import torch
     from tectosaur_amd.pairs_integrator import PairsIntegrator
     pts, tris = small_mesh(True)
