@@ -518,7 +518,7 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
     (+ FP32 direct farfield in the Arnoldi cycles with FP64 refinement) and
     'tree' (+ the FP32 treecode as the inner farfield)."""
     import torch
-    from tectosaur_amd.serve import Server, summarize
+    from tectosaur_amd.serve import PREP_WORKERS, Server, summarize
     from tectosaur_amd.traction_slip import random_requests
     warm = random_requests(2, rate=0.0, seed=seed + 99, n_min=512, n_max=1024)
     for m in modes:
@@ -533,7 +533,8 @@ def serve_bench(n_req, max_tris, rate, seed=0, max_active=32, min_tris=512,
                        ' far, tol 1e-8' % (n_req, rate, min_tris, max_tris)}
     # a mode with the suffix '+nbody' runs the FP32 direct-farfield N-bodies
     # of each tick's solves as one segment-table launch (Server batch_nbody)
-    runs = [('batched_' + m.replace('+', '_'), m, max_active, 4) for m in modes if m != 'seq']
+    runs = [('batched_' + m.replace('+', '_'), m, max_active, PREP_WORKERS) for m in modes
+            if m != 'seq']
     if baseline:
         runs.append(('sequential_f64', 'f64', 1, 0))
     for name, mode, ma, pw in runs:

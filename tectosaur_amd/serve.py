@@ -130,9 +130,14 @@ def _release(a, capture_lock=None, cache_slack=None):
 
 
 _PREP_LOCAL = threading.local()
-# OpenMP threads per preparation worker (four workers x 4 on the 16-core
-# share of one GPU; all 16 per worker measured no faster)
-PREP_OMP_THREADS = 4
+# preparation workers and OpenMP threads per worker on the 16-core share of
+# one GPU: on a 256-request stream of 512-32,768-triangle solves, 2 x 8 ran
+# 43.0-45.9 solves/s (p50 1.7-1.9 s) against 40.1-41.9 (p50 2.2 s) with
+# 4 x 4, 40-41 with 8 x 2 or 6 x 2, 42-43 with 3 x 4, 3 x 5 or 2 x 4, 34
+# with 1 x 16: each request's preparation finishes sooner with more threads,
+# and two workers keep the admissions fed
+PREP_WORKERS = 2
+PREP_OMP_THREADS = 8
 
 
 def _prepare(req, orders, tree=None, omp_threads=None):
@@ -197,7 +202,7 @@ def _finalize(a, now, keep_slip):
 
 class Server:
     def __init__(self, mode='mixed', max_active=32, tol=1e-8, restart=1500, maxiter=6000,
-                 hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=4, tree=None,
+                 hbm_fraction=0.8, orders=None, keep_slip=False, prep_workers=PREP_WORKERS, tree=None,
                  graphs=True, streams=4, switch_interval=1e-3, builders=1, batch_nbody=False,
                  cache_slack=CACHE_SLACK_BYTES):
         # device queries from worker threads start from cached values
