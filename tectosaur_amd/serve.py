@@ -298,6 +298,20 @@ class Server:
         state.stream = stream
         active.append(_Active(rec, prob, op, state, need, stream, build_stream))
 
+    def _adopt_built(self, active, entry, clock):
+        """Adopt a build from the builder queue, entry = (future, record,
+        bytes).  Its objects are unpacked here so that afterwards only the
+        active solve refers to them, and its release (on a builder thread,
+        under the capture lock) is what destroys them and their graphs.
+        Names bound in the serving loop's own frame would keep the last
+        adopted solve alive past its release; rebinding them at the next
+        adoption would then destroy its graphs on the ticking thread without
+        the lock, possibly while a builder captures (a HIP error that aborts
+        the process)."""
+        fut, rec, need = entry
+        prob, op, state, ev, bst = fut.result()
+        self._adopt(active, rec, prob, op, state, need, clock, ev, bst)
+
     def _issue(self, batch):
         """Enqueue one step of every solve in ``batch`` (each on its own
         stream); then, in one native call, the main stream joins those
@@ -461,9 +475,7 @@ class Server:
                     now = clock()
                 # adopt finished builds (block only when nothing else can run)
                 while building and (building[0][0].done() or not active):
-                    fut, rec, need = building.popleft()
-                    prob, op, state, ev, bst = fut.result()
-                    self._adopt(active, rec, prob, op, state, need, clock, ev, bst)
+                    self._adopt_built(active, building.popleft(), clock)
                 if not active:
                     if qi < len(pending):
                         time.sleep(max(0.0, min(0.002, pending[qi].arrival - clock())))

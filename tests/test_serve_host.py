@@ -59,6 +59,54 @@ def test_release_destroys_graphs_under_the_capture_lock_after_stream_join():
     assert not lock.held()
 
 
+def test_adopted_build_is_destroyed_by_its_release_only():
+    # the serving loop adopts finished builds through Server._adopt_built:
+    # after it returns, only the active solve holds the build's objects, so
+    # nothing is destroyed until the release, and the release destroys them
+    # all under the capture lock
+    import collections
+    import concurrent.futures
+    lock = _LockProbe()
+    held = []
+
+    class Prob(_DelProbe):
+        n_tris, pair_evals = 512, 0
+
+    class State(_DelProbe):
+        batch_split = False
+
+    class Rec:
+        admitted = 0.0
+
+    srv = serve.Server.__new__(serve.Server)
+    srv.streams = []
+    srv.group_stream = None
+    srv._next_stream = 0
+    fut = concurrent.futures.Future()
+    fut.set_result((Prob(lock, held), _DelProbe(lock, held), State(lock, held), None, None))
+    building = collections.deque([(fut, Rec(), 5)])
+    del fut
+    active = []
+    srv._adopt_built(active, building.popleft(), lambda: 1.0)
+    assert held == [] and len(active) == 1
+    serve._release(active[0], lock)
+    assert held == [True, True, True]
+    assert not lock.held()
+    # the pattern it replaces: the build unpacked in the caller's frame
+    # outlives the release, and dropping those names destroys it unlocked
+    held.clear()
+    fut = concurrent.futures.Future()
+    fut.set_result((Prob(lock, held), _DelProbe(lock, held), State(lock, held), None, None))
+    prob, op, state, ev, bst = fut.result()
+    del fut
+    active = []
+    srv._adopt(active, Rec(), prob, op, state, 5, lambda: 1.0, ev, bst)
+    serve._release(active[0], lock)
+    assert held == []
+    del prob, op, state
+    assert held == [False, False, False]
+
+
 def test_release_empties_the_allocator_cache_beyond_the_slack(monkeypatch):
     # destroyed graphs leave their memory pools reserved: a release empties
     # the cache, under the capture lock, once the unallocated reserved
